@@ -1,0 +1,172 @@
+/*
+ * pgeof_hip.h -- C ABI of the MI355X (gfx950) implementation of pgeof's hot path.
+ *
+ * Library: point_geometric_features_amd/libpgeof_hip.so (HIP kernels + host runtime).
+ * All functions return PGEOF_OK (0) or an error code; pgeof_last_error() gives
+ * the message of the calling thread's last failure.  No function here takes or
+ * returns a torch or numpy type: pointers, sizes and a HIP stream (void*).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to
+ * the pgeof repository, drprojects/point_geometric_features).
+ */
+#ifndef PGEOF_HIP_H
+#define PGEOF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGEOF_OK 0
+#define PGEOF_EINVAL 1   /* reference throws std::invalid_argument -> Python ValueError */
+#define PGEOF_ERUNTIME 2 /* HIP runtime failure (no device, launch failure, ...) */
+#define PGEOF_ENOMEM 3   /* device or workspace memory exhausted */
+
+/* Message of the calling thread's last error ("" if none). */
+const char* pgeof_last_error(void);
+
+/* ABI version (major * 100 + minor). */
+int pgeof_abi_version(void);
+
+/* Number of HIP devices visible to the process (0 without a GPU). */
+int pgeof_device_count(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host-memory entry points.  Inputs and outputs are host arrays; the call    */
+/* stages them to the current HIP device on a per-thread stream and returns   */
+/* when the outputs are written.  Thread-safe; the Python binding calls these */
+/* with the GIL released.                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces pgeof::nanoflann_knn_search<float> (include/nn_search.hpp:31-67),
+ * bound as pgeof.knn_search (src/pgeof_ext.cpp:118).
+ * data (n_data,3) f32 row-major; query (n_query,3) f32 (may alias data).
+ * indices/sqr_dist: (n_query, knn) outputs, rows sorted by (sqr_dist, index).
+ * EINVAL if knn > n_data (nn_search.hpp:37). */
+int pgeof_knn_search(const float* data, int64_t n_data, const float* query, int64_t n_query, uint32_t knn,
+                     uint32_t* indices, float* sqr_dist);
+
+/* Replaces pgeof::nanoflann_radius_search<float> (include/nn_search.hpp:85-132),
+ * bound as pgeof.radius_search (src/pgeof_ext.cpp:130-132).
+ * indices (n_query, max_knn) i32 padded with -1; sqr_dist padded with 0 except
+ * that a row with fewer than max_knn hits holds r*r in its last column
+ * (nanoflann RKNNResultSet::init).  Hits are d^2 < r*r, nearest first.
+ * EINVAL if max_knn > n_data (nn_search.hpp:92-95). */
+int pgeof_radius_search(const float* data, int64_t n_data, const float* query, int64_t n_query,
+                        float search_radius, uint32_t max_knn, int32_t* indices, float* sqr_dist);
+
+/* Replaces pgeof::compute_geometric_features<float,11> (include/pgeof.hpp:75-117),
+ * bound as pgeof.compute_features (src/pgeof_ext.cpp:34-36).
+ * nn (n_nn,) u32, nn_ptr (n_nn_ptr,) u32 CSR; features (n_nn_ptr-1, 11) f32.
+ * EINVAL if k_min < 1 (pgeof.hpp:81), if n_nn_ptr < 1, or if the CSR indexes
+ * outside nn or xyz (undefined behaviour in the reference). */
+int pgeof_compute_features(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                           const uint32_t* nn_ptr, int64_t n_nn_ptr, uint64_t k_min, float* features);
+
+/* Replaces pgeof::compute_geometric_features_multiscale<float,11>
+ * (include/pgeof.hpp:159-211), bound as pgeof.compute_features_multiscale
+ * (src/pgeof_ext.cpp:59-61).  features (n_nn_ptr-1, n_scales, 11) f32.
+ * EINVAL unless k_scales is non-decreasing with every entry >= 1 (pgeof.hpp:123-132, 165-168). */
+int pgeof_compute_features_multiscale(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                      const uint32_t* nn_ptr, int64_t n_nn_ptr, const uint32_t* k_scales,
+                                      int64_t n_scales, float* features);
+
+/* Replaces pgeof::compute_geometric_features_optimal<float,12>
+ * (include/pgeof.hpp:243-310), bound as pgeof.compute_features_optimal
+ * (src/pgeof_ext.cpp:86-89).  features (n_nn_ptr-1, 12) f32, column 11 = k_opt.
+ * EINVAL if k_min < 1 && k_min_search < 1 (pgeof.hpp:250), or k_step < 1
+ * (a modulo by zero in the reference). */
+int pgeof_compute_features_optimal(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                   const uint32_t* nn_ptr, int64_t n_nn_ptr, uint32_t k_min, uint32_t k_step,
+                                   uint32_t k_min_search, float* features);
+
+/* Replace pgeof::compute_geometric_features_selected<float|double>
+ * (include/pgeof.hpp:325-375, include/pca.hpp:212-295), bound as the two
+ * overloads of pgeof.compute_features_selected (src/pgeof_ext.cpp:148-177).
+ * selected: EFeatureID values (include/pca.hpp:47-63); features (n, n_selected). */
+int pgeof_compute_features_selected_f32(const float* xyz, int64_t n, float search_radius, uint32_t max_knn,
+                                        const int32_t* selected, int64_t n_selected, float* features);
+int pgeof_compute_features_selected_f64(const double* xyz, int64_t n, double search_radius, uint32_t max_knn,
+                                        const int32_t* selected, int64_t n_selected, double* features);
+
+/* ------------------------------------------------------------------------ */
+/* Device-memory entry points.  All pointers are device pointers on the       */
+/* current HIP device; work is ordered on `stream` (a hipStream_t, NULL = the */
+/* null stream).  `workspace` may be NULL: the call then takes and returns    */
+/* its scratch with hipMallocAsync/hipFreeAsync on `stream`.  The search      */
+/* entry points read the point cloud's bounding box back to the host, so they */
+/* synchronise `stream` once or twice before their last kernel is queued.     */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace bytes for the search entry points below. */
+size_t pgeof_knn_workspace_size(int64_t n_data, int64_t n_query, uint32_t knn);
+size_t pgeof_radius_workspace_size(int64_t n_data, int64_t n_query, uint32_t max_knn);
+
+/* Device form of pgeof_knn_search.  sqr_dist may be NULL. */
+int pgeof_knn_search_device(const float* data, int64_t n_data, const float* query, int64_t n_query, uint32_t knn,
+                            uint32_t* indices, float* sqr_dist, void* workspace, size_t workspace_bytes,
+                            void* stream);
+
+/* Device form of pgeof_radius_search. */
+int pgeof_radius_search_device(const float* data, int64_t n_data, const float* query, int64_t n_query,
+                               float search_radius, uint32_t max_knn, int32_t* indices, float* sqr_dist,
+                               void* workspace, size_t workspace_bytes, void* stream);
+
+/* knn_search(xyz, xyz, knn) -> CSR (README.md:136-141: nn = idx.flatten(),
+ * nn_ptr = arange(n+1)*knn) -> compute_features(xyz, nn, nn_ptr, k_min=1),
+ * fused: the features are computed from the search's final neighbour lists
+ * without re-reading nn.  nn (n*knn) u32, nn_ptr (n+1) u32, features (n,11);
+ * sqr_dist (n*knn) may be NULL.  Results equal the two-step path bit for bit. */
+int pgeof_knn_features_device(const float* xyz, int64_t n, uint32_t knn, uint32_t* nn, uint32_t* nn_ptr,
+                              float* sqr_dist, float* features, void* workspace, size_t workspace_bytes,
+                              void* stream);
+
+/* radius_search(xyz, xyz, r, max_knn) -> CSR of the hits (README.md:158-159:
+ * nn_ptr = [0, cumsum(count)], nn = idx[idx >= 0]).  nn must hold n*max_knn
+ * entries; nn_ptr (n+1).  *nnz_out (host) receives nn_ptr[n]. */
+size_t pgeof_radius_csr_workspace_size(int64_t n, uint32_t max_knn);
+int pgeof_radius_csr_device(const float* xyz, int64_t n, float search_radius, uint32_t max_knn, uint32_t* nn,
+                            uint32_t* nn_ptr, int64_t* nnz_out, void* workspace, size_t workspace_bytes,
+                            void* stream);
+
+/* Device forms of the CSR feature drivers.  If err is non-NULL (a device u32)
+ * the call is fully asynchronous and ORs 1 (bad neighbour index) / 2 (bad
+ * nn_ptr) into *err; if NULL the call synchronises `stream` and returns
+ * PGEOF_EINVAL on such input.  Argument checks are those of the host forms. */
+int pgeof_compute_features_device(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                  const uint32_t* nn_ptr, int64_t n_nn_ptr, uint64_t k_min, float* features,
+                                  uint32_t* err, void* stream);
+int pgeof_compute_features_multiscale_device(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                             const uint32_t* nn_ptr, int64_t n_nn_ptr,
+                                             const uint32_t* k_scales_host, int64_t n_scales, float* features,
+                                             uint32_t* err, void* stream);
+int pgeof_compute_features_optimal_device(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                          const uint32_t* nn_ptr, int64_t n_nn_ptr, uint32_t k_min,
+                                          uint32_t k_step, uint32_t k_min_search, float* features, uint32_t* err,
+                                          void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Tuning and profiling (no counterpart in the reference).                   */
+/* ------------------------------------------------------------------------ */
+
+/* Set a tuning knob: "knn_occupancy" (target points per grid cell, as a
+ * fraction of k), "grid_cells_per_point" (cap on grid cells / points),
+ * "lds_budget" (bytes of LDS per workgroup for the search heaps),
+ * "radius_cell_factor" (grid cell edge / search radius). */
+int pgeof_set_tuning(const char* key, double value);
+double pgeof_get_tuning(const char* key);
+
+/* When enabled, every kernel launch is bracketed by HIP events on its stream.
+ * pgeof_profile_report writes {"kernel": [launches, total_ms], ...} as JSON
+ * into buf (synchronising the recorded events) and returns the length. */
+void pgeof_profile_enable(int on);
+int pgeof_profile_report(char* buf, size_t len);
+void pgeof_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGEOF_HIP_H */

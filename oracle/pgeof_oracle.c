@@ -1,0 +1,1043 @@
+/*
+ * pgeof_oracle.c -- CPU restatement of pgeof's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the
+ * smoke() entry and bench.py's cpu_baseline leg may load it.  The product
+ * (point_geometric_features_amd) never links or calls it.
+ *
+ * What it restates (reference paths relative to the pgeof repository):
+ *   - include/pca.hpp:71-98    pca_from_pointcloud   -> oracle_pca()
+ *       mean, centred covariance / k, symmetric eigensolve in ASCENDING order
+ *       (Eigen's SelfAdjointEigenSolver convention), clamp >= 0, flip v2 if
+ *       v2.z < 0.  Here in FP64 with a cyclic Jacobi eigensolver (Golub & Van
+ *       Loan, "Matrix Computations", sec. 8.5) and Eigen's selection-sort
+ *       ordering of the eigenpairs.
+ *   - include/pca.hpp:113-129  pca_from_neighborhood (gather of CSR rows)
+ *   - include/pca.hpp:140-150  compute_eigentropy     -> eigentropy32()
+ *   - include/pca.hpp:160-200  compute_features       -> features_from_pca()
+ *   - include/pca.hpp:212-295  compute_selected_features -> oracle_selected()
+ *   - include/pgeof.hpp:75-117   compute_geometric_features        -> oracle_compute_features()
+ *   - include/pgeof.hpp:123-211  check_scales + ..._multiscale     -> oracle_compute_features_multiscale()
+ *   - include/pgeof.hpp:243-310  ..._optimal                       -> oracle_compute_features_optimal()
+ *   - include/pgeof.hpp:325-375  ..._selected                      -> oracle_compute_features_selected()
+ *   - include/nn_search.hpp:31-67   nanoflann_knn_search    -> oracle_knn_bruteforce(), oracle_knn_kdtree()
+ *   - include/nn_search.hpp:85-132  nanoflann_radius_search -> oracle_radius_bruteforce(), oracle_radius_kdtree()
+ *       nanoflann (un-vendored, unpinned version) computes the squared L2
+ *       distance in the point type with metric_L2_Simple:
+ *       d = 0; d += dx*dx; d += dy*dy; d += dz*dz, dx = query - point,
+ *       and keeps a candidate iff d < worst (strict).  Exact ties are broken
+ *       here by the smaller point index (nanoflann: traversal order).
+ *
+ * Build: oracle/Makefile (gcc, -ffp-contract=off so no FMA contraction,
+ * as in the reference's x86-64 baseline build without -march).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORACLE_EPS 1e-3
+
+/* ------------------------------------------------------------------------ */
+/* FP64 symmetric 3x3 eigensolver: cyclic Jacobi, then Eigen's sort.         */
+/* A packed as a00 a01 a02 a11 a12 a22.  V[r*3+c]: column c = eigenvector c. */
+/* ------------------------------------------------------------------------ */
+static void jacobi3(const double *Ap, double w[3], double V[9])
+{
+    double a[3][3] = {{Ap[0], Ap[1], Ap[2]}, {Ap[1], Ap[3], Ap[4]}, {Ap[2], Ap[4], Ap[5]}};
+    double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    static const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+    for (int sweep = 0; sweep < 100; ++sweep)
+    {
+        double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        double dg = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (off == 0.0 || off <= 1e-36 * dg) break;
+        for (int m = 0; m < 3; ++m)
+        {
+            int p = P[m], q = Q[m];
+            double apq = a[p][q];
+            if (apq == 0.0) continue;
+            double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+            double t;
+            if (fabs(theta) > 1e150)
+                t = 0.5 / theta;
+            else
+                t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+            for (int r = 0; r < 3; ++r)
+            {
+                if (r == p || r == q) continue;
+                double arp = a[r][p], arq = a[r][q];
+                a[r][p] = a[p][r] = c * arp - s * arq;
+                a[r][q] = a[q][r] = s * arp + c * arq;
+            }
+            a[p][p] -= t * apq;
+            a[q][q] += t * apq;
+            a[p][q] = a[q][p] = 0.0;
+            for (int r = 0; r < 3; ++r)
+            {
+                double vrp = v[r][p], vrq = v[r][q];
+                v[r][p] = c * vrp - s * vrq;
+                v[r][q] = s * vrp + c * vrq;
+            }
+        }
+    }
+    for (int i = 0; i < 3; ++i) w[i] = a[i][i];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) V[r * 3 + c] = v[r][c];
+    /* Eigen's SelfAdjointEigenSolver ordering: for each i, find the FIRST
+       minimum of w[i..2] and swap it (and its column) into position i. */
+    for (int i = 0; i < 2; ++i)
+    {
+        int k = i;
+        for (int j = i + 1; j < 3; ++j)
+            if (w[j] < w[k]) k = j;
+        if (k != i)
+        {
+            double tw = w[i];
+            w[i] = w[k];
+            w[k] = tw;
+            for (int r = 0; r < 3; ++r)
+            {
+                double tv = V[r * 3 + i];
+                V[r * 3 + i] = V[r * 3 + k];
+                V[r * 3 + k] = tv;
+            }
+        }
+    }
+}
+
+/* PCA result in FP64: val ascending (clamped >= 0), v0 v1 v2 unit vectors.  */
+typedef struct
+{
+    double val[3];
+    double v[3][3]; /* v[i] = eigenvector of val[i] */
+} pca64_t;
+
+/* Orientation of v2 (pca.hpp:96): flip when v2.z < 0.  When v2.z == 0 the
+   reference keeps the solver's sign (not pinned); this build, GPU and oracle
+   alike, then makes the first non-zero of (y, x) positive. */
+static void orient_v2(double v2[3])
+{
+    int flip = 0;
+    if (v2[2] < 0.0)
+        flip = 1;
+    else if (v2[2] == 0.0)
+    {
+        if (v2[1] < 0.0)
+            flip = 1;
+        else if (v2[1] == 0.0 && v2[0] < 0.0)
+            flip = 1;
+    }
+    if (flip)
+        for (int c = 0; c < 3; ++c) v2[c] = -v2[c];
+}
+
+/* pca.hpp:71-98, from an FP64 covariance. */
+static void pca_from_cov(const double cov[6], pca64_t *out)
+{
+    double w[3], V[9];
+    jacobi3(cov, w, V);
+    for (int i = 0; i < 3; ++i)
+    {
+        out->val[i] = w[i] > 0.0 ? w[i] : 0.0;
+        for (int c = 0; c < 3; ++c) out->v[i][c] = V[c * 3 + i];
+    }
+    orient_v2(out->v[2]);
+}
+
+/* Two-pass centred covariance of k float points, in FP64 (pca.hpp:75-76). */
+static void cov_two_pass(const float *xyz, const uint32_t *idx, size_t k, double cov[6])
+{
+    double m[3] = {0, 0, 0};
+    for (size_t j = 0; j < k; ++j)
+        for (int c = 0; c < 3; ++c) m[c] += (double)xyz[(size_t)idx[j] * 3 + c];
+    for (int c = 0; c < 3; ++c) m[c] /= (double)k;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t j = 0; j < k; ++j)
+    {
+        double d0 = (double)xyz[(size_t)idx[j] * 3 + 0] - m[0];
+        double d1 = (double)xyz[(size_t)idx[j] * 3 + 1] - m[1];
+        double d2 = (double)xyz[(size_t)idx[j] * 3 + 2] - m[2];
+        s[0] += d0 * d0;
+        s[1] += d0 * d1;
+        s[2] += d0 * d2;
+        s[3] += d1 * d1;
+        s[4] += d1 * d2;
+        s[5] += d2 * d2;
+    }
+    for (int i = 0; i < 6; ++i) cov[i] = s[i] / (double)k;
+}
+
+static void cov_two_pass_f64(const double *xyz, const uint32_t *idx, size_t k, double cov[6])
+{
+    double m[3] = {0, 0, 0};
+    for (size_t j = 0; j < k; ++j)
+        for (int c = 0; c < 3; ++c) m[c] += xyz[(size_t)idx[j] * 3 + c];
+    for (int c = 0; c < 3; ++c) m[c] /= (double)k;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t j = 0; j < k; ++j)
+    {
+        double d0 = xyz[(size_t)idx[j] * 3 + 0] - m[0];
+        double d1 = xyz[(size_t)idx[j] * 3 + 1] - m[1];
+        double d2 = xyz[(size_t)idx[j] * 3 + 2] - m[2];
+        s[0] += d0 * d0;
+        s[1] += d0 * d1;
+        s[2] += d0 * d2;
+        s[3] += d1 * d1;
+        s[4] += d1 * d2;
+        s[5] += d2 * d2;
+    }
+    for (int i = 0; i < 6; ++i) cov[i] = s[i] / (double)k;
+}
+
+/* pca.hpp:160-200 in FP64.  f[11] must be zeroed by the caller. */
+static void features_from_pca(const pca64_t *p, double *f)
+{
+    const double sq_eps = 1e-6, cub_eps = 1e-9, one_third = 1.0 / 3.0;
+    double s0 = sqrt(p->val[0]), s1 = sqrt(p->val[1]), s2 = sqrt(p->val[2]);
+    double fact = 1.0 / (s0 + ORACLE_EPS);
+    f[4] = p->v[2][0];
+    f[5] = p->v[2][1];
+    f[6] = p->v[2][2];
+    f[0] = (s0 - s1) * fact;
+    f[1] = (s1 - s2) * fact;
+    f[2] = s2 * fact;
+    f[7] = s0;
+    f[8] = sqrt(s0 * s1 + sq_eps);
+    f[9] = pow(s0 * s1 * s2 + cub_eps, one_third);
+    f[10] = s2 / (s0 + s1 + s2 + ORACLE_EPS);
+    if (s0 > 0.0)
+    {
+        double u[3];
+        for (int c = 0; c < 3; ++c)
+            u[c] = p->val[0] * fabs(p->v[0][c]) + p->val[1] * fabs(p->v[1][c]) + p->val[2] * fabs(p->v[2][c]);
+        f[3] = u[2] / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    }
+}
+
+/* pca.hpp:140-150, evaluated in FP32 on the FP32-rounded eigenvalues, as the
+   reference's <float> instantiation does. */
+static float eigentropy32(const double val64[3])
+{
+    const float eps = 1e-3f;
+    float v0 = (float)val64[0], v1 = (float)val64[1], v2 = (float)val64[2];
+    float sum = v0 + v1 + v2 + eps;
+    float e0 = v0 / sum, e1 = v1 / sum, e2 = v2 / sum;
+    return -e0 * logf(e0 + eps) - e1 * logf(e1 + eps) - e2 * logf(e2 + eps);
+}
+
+static double eigentropy64(const double val[3])
+{
+    double sum = val[0] + val[1] + val[2] + ORACLE_EPS;
+    double h = 0.0;
+    for (int i = 0; i < 3; ++i)
+    {
+        double e = val[i] / sum;
+        h -= e * log(e + ORACLE_EPS);
+    }
+    return h;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Public single-neighbourhood entry points (used by the tests).            */
+/* ------------------------------------------------------------------------ */
+
+/* PCA of k float points given by index: val[3] ascending, vec[9] with vec[i*3+c]
+   = component c of eigenvector i. */
+void oracle_pca(const float *xyz, const uint32_t *idx, int64_t k, double *val, double *vec)
+{
+    double cov[6];
+    pca64_t p;
+    cov_two_pass(xyz, idx, (size_t)k, cov);
+    pca_from_cov(cov, &p);
+    for (int i = 0; i < 3; ++i)
+    {
+        val[i] = p.val[i];
+        for (int c = 0; c < 3; ++c) vec[i * 3 + c] = p.v[i][c];
+    }
+}
+
+/* Eigen-decomposition of a packed covariance (for known-answer tests). */
+void oracle_eig3(const double *cov6, double *val, double *vec)
+{
+    pca64_t p;
+    pca_from_cov(cov6, &p);
+    for (int i = 0; i < 3; ++i)
+    {
+        val[i] = p.val[i];
+        for (int c = 0; c < 3; ++c) vec[i * 3 + c] = p.v[i][c];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Drivers over CSR (pgeof.hpp).  Return 0 on success, 1 on a ValueError     */
+/* condition of the reference, 2 on an input this build rejects (bad CSR).   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct
+{
+    const float *xyz;
+    int64_t n_xyz;
+    const uint32_t *nn;
+    int64_t n_nn;
+    const uint32_t *nn_ptr;
+    int64_t n_points;
+    /* plain / optimal */
+    uint32_t k_min, k_step, k_min_search;
+    int eigentropy_fp32;
+    /* multiscale */
+    const uint32_t *scales;
+    int64_t n_scales;
+    double *out;
+    int mode; /* 0 plain, 1 multiscale, 2 optimal */
+    int n_threads;
+    int tid;
+} csr_job_t;
+
+static int check_csr(const float *xyz, int64_t n_xyz, const uint32_t *nn, int64_t n_nn, const uint32_t *nn_ptr,
+                     int64_t n_points)
+{
+    (void)xyz;
+    for (int64_t i = 0; i < n_points; ++i)
+    {
+        if (nn_ptr[i + 1] < nn_ptr[i]) return 2;
+        if ((int64_t)nn_ptr[i + 1] > n_nn) return 2;
+    }
+    for (int64_t i = 0; i < n_points; ++i)
+        for (uint32_t j = nn_ptr[i]; j < nn_ptr[i + 1]; ++j)
+            if ((int64_t)nn[j] >= n_xyz) return 2;
+    return 0;
+}
+
+static void *csr_worker(void *arg)
+{
+    csr_job_t *J = (csr_job_t *)arg;
+    for (int64_t i = J->tid; i < J->n_points; i += J->n_threads)
+    {
+        const size_t k_nn = (size_t)(J->nn_ptr[i + 1] - J->nn_ptr[i]);
+        const uint32_t *row = J->nn + J->nn_ptr[i];
+        double cov[6];
+        pca64_t p;
+        if (J->mode == 0)
+        {
+            /* pgeof.hpp:99-107 */
+            if (k_nn >= J->k_min)
+            {
+                cov_two_pass(J->xyz, row, k_nn, cov);
+                pca_from_cov(cov, &p);
+                features_from_pca(&p, J->out + (size_t)i * 11);
+            }
+        }
+        else if (J->mode == 1)
+        {
+            /* pgeof.hpp:186-199 */
+            for (int64_t s = 0; s < J->n_scales; ++s)
+            {
+                const size_t ks = J->scales[s];
+                if (k_nn < ks) break;
+                cov_two_pass(J->xyz, row, ks, cov);
+                pca_from_cov(cov, &p);
+                features_from_pca(&p, J->out + ((size_t)i * (size_t)J->n_scales + (size_t)s) * 11);
+            }
+        }
+        else
+        {
+            /* pgeof.hpp:268-299 */
+            if (k_nn >= J->k_min && k_nn >= J->k_min_search)
+            {
+                size_t k0 = J->k_min > J->k_min_search ? J->k_min : J->k_min_search;
+                if (k0 > k_nn) k0 = k_nn;
+                pca64_t best;
+                double h_best = 1.0;
+                float h_best32 = 1.0f;
+                size_t k_opt = k_nn;
+                for (size_t k = k0; k <= k_nn; ++k)
+                {
+                    if ((k > k0) && (k % J->k_step != 0) && (k != k_nn)) continue;
+                    cov_two_pass(J->xyz, row, k, cov);
+                    pca_from_cov(cov, &p);
+                    int better;
+                    if (J->eigentropy_fp32)
+                    {
+                        float h = eigentropy32(p.val);
+                        better = (k == k0) || (h < h_best32);
+                        if (better) h_best32 = h;
+                    }
+                    else
+                    {
+                        double h = eigentropy64(p.val);
+                        better = (k == k0) || (h < h_best);
+                        if (better) h_best = h;
+                    }
+                    if (better)
+                    {
+                        k_opt = k;
+                        best = p;
+                    }
+                }
+                double *f = J->out + (size_t)i * 12;
+                features_from_pca(&best, f);
+                f[11] = (double)k_opt;
+            }
+        }
+    }
+    return NULL;
+}
+
+static int run_csr(csr_job_t *base, int n_threads)
+{
+    if (n_threads < 1) n_threads = 1;
+    if (n_threads > 256) n_threads = 256;
+    pthread_t th[256];
+    csr_job_t jobs[256];
+    for (int t = 0; t < n_threads; ++t)
+    {
+        jobs[t] = *base;
+        jobs[t].n_threads = n_threads;
+        jobs[t].tid = t;
+    }
+    for (int t = 1; t < n_threads; ++t) pthread_create(&th[t], NULL, csr_worker, &jobs[t]);
+    csr_worker(&jobs[0]);
+    for (int t = 1; t < n_threads; ++t) pthread_join(th[t], NULL);
+    return 0;
+}
+
+/* pgeof.hpp:75-117.  out: (n_points, 11) doubles, written fully. */
+int oracle_compute_features(const float *xyz, int64_t n_xyz, const uint32_t *nn, int64_t n_nn, const uint32_t *nn_ptr,
+                            int64_t n_ptr, uint32_t k_min, double *out, int n_threads)
+{
+    if (k_min < 1) return 1; /* pgeof.hpp:81 */
+    if (n_ptr < 1) return 2;
+    int64_t n_points = n_ptr - 1;
+    if (check_csr(xyz, n_xyz, nn, n_nn, nn_ptr, n_points)) return 2;
+    memset(out, 0, sizeof(double) * (size_t)n_points * 11);
+    csr_job_t J;
+    memset(&J, 0, sizeof J);
+    J.xyz = xyz, J.n_xyz = n_xyz, J.nn = nn, J.n_nn = n_nn, J.nn_ptr = nn_ptr, J.n_points = n_points;
+    J.k_min = k_min, J.out = out, J.mode = 0;
+    return run_csr(&J, n_threads);
+}
+
+/* pgeof.hpp:123-132 */
+static int check_scales(const uint32_t *s, int64_t n)
+{
+    uint32_t prev = 1;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        if (s[i] < prev) return 0;
+        prev = s[i];
+    }
+    return 1;
+}
+
+/* pgeof.hpp:159-211.  out: (n_points, n_scales, 11). */
+int oracle_compute_features_multiscale(const float *xyz, int64_t n_xyz, const uint32_t *nn, int64_t n_nn,
+                                       const uint32_t *nn_ptr, int64_t n_ptr, const uint32_t *scales,
+                                       int64_t n_scales, double *out, int n_threads)
+{
+    if (!check_scales(scales, n_scales)) return 1;
+    if (n_ptr < 1) return 2;
+    int64_t n_points = n_ptr - 1;
+    if (check_csr(xyz, n_xyz, nn, n_nn, nn_ptr, n_points)) return 2;
+    memset(out, 0, sizeof(double) * (size_t)n_points * (size_t)n_scales * 11);
+    csr_job_t J;
+    memset(&J, 0, sizeof J);
+    J.xyz = xyz, J.n_xyz = n_xyz, J.nn = nn, J.n_nn = n_nn, J.nn_ptr = nn_ptr, J.n_points = n_points;
+    J.scales = scales, J.n_scales = n_scales, J.out = out, J.mode = 1;
+    return run_csr(&J, n_threads);
+}
+
+/* pgeof.hpp:243-310.  out: (n_points, 12).  eigentropy_fp32 = 1 evaluates the
+   entropy comparison in FP32 (as the reference's <float> build); 0 in FP64. */
+int oracle_compute_features_optimal(const float *xyz, int64_t n_xyz, const uint32_t *nn, int64_t n_nn,
+                                    const uint32_t *nn_ptr, int64_t n_ptr, uint32_t k_min, uint32_t k_step,
+                                    uint32_t k_min_search, int eigentropy_fp32, double *out, int n_threads)
+{
+    if (k_min < 1 && k_min_search < 1) return 1; /* pgeof.hpp:250 */
+    if (k_step < 1) return 2;                    /* modulo by zero in the reference */
+    if (n_ptr < 1) return 2;
+    int64_t n_points = n_ptr - 1;
+    if (check_csr(xyz, n_xyz, nn, n_nn, nn_ptr, n_points)) return 2;
+    memset(out, 0, sizeof(double) * (size_t)n_points * 12);
+    csr_job_t J;
+    memset(&J, 0, sizeof J);
+    J.xyz = xyz, J.n_xyz = n_xyz, J.nn = nn, J.n_nn = n_nn, J.nn_ptr = nn_ptr, J.n_points = n_points;
+    J.k_min = k_min, J.k_step = k_step, J.k_min_search = k_min_search, J.eigentropy_fp32 = eigentropy_fp32;
+    J.out = out, J.mode = 2;
+    return run_csr(&J, n_threads);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Neighbour search.                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* nanoflann metric_L2_Simple in float, no contraction. */
+static inline float sqdist_f32(const float *q, const float *p)
+{
+    float d = 0.0f;
+    float t;
+    t = q[0] - p[0];
+    d += t * t;
+    t = q[1] - p[1];
+    d += t * t;
+    t = q[2] - p[2];
+    d += t * t;
+    return d;
+}
+static inline double sqdist_f64(const double *q, const double *p)
+{
+    double d = 0.0, t;
+    t = q[0] - p[0];
+    d += t * t;
+    t = q[1] - p[1];
+    d += t * t;
+    t = q[2] - p[2];
+    d += t * t;
+    return d;
+}
+
+/* Bounded max-heap on (dist, index) pairs, lexicographic. */
+typedef struct
+{
+    double *d;
+    uint32_t *i;
+    int64_t cap, n;
+} bheap_t;
+
+static inline int pair_less(double da, uint32_t ia, double db, uint32_t ib)
+{
+    return da < db || (da == db && ia < ib);
+}
+
+static void bheap_push(bheap_t *h, double d, uint32_t idx)
+{
+    if (h->n < h->cap)
+    {
+        int64_t c = h->n++;
+        while (c > 0)
+        {
+            int64_t p = (c - 1) / 2;
+            if (!pair_less(h->d[p], h->i[p], d, idx)) break;
+            h->d[c] = h->d[p];
+            h->i[c] = h->i[p];
+            c = p;
+        }
+        h->d[c] = d;
+        h->i[c] = idx;
+        return;
+    }
+    if (!pair_less(d, idx, h->d[0], h->i[0])) return;
+    int64_t c = 0;
+    for (;;)
+    {
+        int64_t l = 2 * c + 1, r = l + 1, m = c;
+        double md = d;
+        uint32_t mi = idx;
+        if (l < h->n && pair_less(md, mi, h->d[l], h->i[l])) m = l, md = h->d[l], mi = h->i[l];
+        if (r < h->n && pair_less(md, mi, h->d[r], h->i[r])) m = r, md = h->d[r], mi = h->i[r];
+        if (m == c) break;
+        h->d[c] = h->d[m];
+        h->i[c] = h->i[m];
+        c = m;
+    }
+    h->d[c] = d;
+    h->i[c] = idx;
+}
+
+/* pop everything into ascending arrays */
+static void bheap_drain(bheap_t *h, double *d_out, uint32_t *i_out)
+{
+    int64_t n = h->n;
+    for (int64_t j = n - 1; j >= 0; --j)
+    {
+        d_out[j] = h->d[0];
+        i_out[j] = h->i[0];
+        /* remove root */
+        double ld = h->d[h->n - 1];
+        uint32_t li = h->i[h->n - 1];
+        h->n--;
+        int64_t c = 0;
+        for (;;)
+        {
+            int64_t l = 2 * c + 1, r = l + 1, m = c;
+            double md = ld;
+            uint32_t mi = li;
+            if (l < h->n && pair_less(md, mi, h->d[l], h->i[l])) m = l, md = h->d[l], mi = h->i[l];
+            if (r < h->n && pair_less(md, mi, h->d[r], h->i[r])) m = r, md = h->d[r], mi = h->i[r];
+            if (m == c) break;
+            h->d[c] = h->d[m];
+            h->i[c] = h->i[m];
+            c = m;
+        }
+        if (h->n > 0)
+        {
+            h->d[c] = ld;
+            h->i[c] = li;
+        }
+    }
+}
+
+/* ---- KD-tree (median split on the widest dimension, leaf size 10, like the
+   reference's kd_tree_t(3, data, 10, 0) at nn_search.hpp:39) ---- */
+typedef struct
+{
+    int32_t left, right; /* children, -1 for leaves */
+    int32_t dim;
+    float split;
+    int64_t lo, hi; /* index range in perm for leaves */
+    float bmin[3], bmax[3];
+} kdnode_t;
+
+typedef struct
+{
+    const float *pts;
+    int64_t n;
+    uint32_t *perm;
+    kdnode_t *nodes;
+    int64_t n_nodes, cap_nodes;
+} kdtree_t;
+
+static void kd_select(const float *pts, uint32_t *perm, int64_t lo, int64_t hi, int64_t kth, int dim)
+{
+    /* quickselect on perm[lo..hi) by coordinate dim, ties by index */
+    while (hi - lo > 1)
+    {
+        int64_t mid = lo + (hi - lo) / 2;
+        uint32_t piv = perm[mid];
+        float pv = pts[(size_t)piv * 3 + dim];
+        /* three-way partition */
+        int64_t lt = lo, gt = hi - 1, i = lo;
+        while (i <= gt)
+        {
+            uint32_t e = perm[i];
+            float ev = pts[(size_t)e * 3 + dim];
+            if (ev < pv || (ev == pv && e < piv))
+            {
+                perm[i] = perm[lt];
+                perm[lt] = e;
+                lt++, i++;
+            }
+            else if (ev > pv || (ev == pv && e > piv))
+            {
+                perm[i] = perm[gt];
+                perm[gt] = e;
+                gt--;
+            }
+            else
+                i++;
+        }
+        if (kth < lt)
+            hi = lt;
+        else if (kth > gt)
+            lo = gt + 1;
+        else
+            return;
+    }
+}
+
+static int32_t kd_build_rec(kdtree_t *T, int64_t lo, int64_t hi)
+{
+    if (T->n_nodes == T->cap_nodes)
+    {
+        T->cap_nodes *= 2;
+        T->nodes = (kdnode_t *)realloc(T->nodes, sizeof(kdnode_t) * (size_t)T->cap_nodes);
+    }
+    int32_t id = (int32_t)T->n_nodes++;
+    kdnode_t nd;
+    memset(&nd, 0, sizeof nd);
+    nd.lo = lo, nd.hi = hi, nd.left = nd.right = -1;
+    for (int c = 0; c < 3; ++c) nd.bmin[c] = INFINITY, nd.bmax[c] = -INFINITY;
+    for (int64_t j = lo; j < hi; ++j)
+        for (int c = 0; c < 3; ++c)
+        {
+            float v = T->pts[(size_t)T->perm[j] * 3 + c];
+            if (v < nd.bmin[c]) nd.bmin[c] = v;
+            if (v > nd.bmax[c]) nd.bmax[c] = v;
+        }
+    if (hi - lo > 10)
+    {
+        int dim = 0;
+        float w = nd.bmax[0] - nd.bmin[0];
+        for (int c = 1; c < 3; ++c)
+            if (nd.bmax[c] - nd.bmin[c] > w) w = nd.bmax[c] - nd.bmin[c], dim = c;
+        int64_t mid = lo + (hi - lo) / 2;
+        kd_select(T->pts, T->perm, lo, hi, mid, dim);
+        nd.dim = dim;
+        nd.split = T->pts[(size_t)T->perm[mid] * 3 + dim];
+        T->nodes[id] = nd;
+        int32_t l = kd_build_rec(T, lo, mid);
+        int32_t r = kd_build_rec(T, mid, hi);
+        T->nodes[id].left = l;
+        T->nodes[id].right = r;
+        return id;
+    }
+    T->nodes[id] = nd;
+    return id;
+}
+
+static void kd_build(kdtree_t *T, const float *pts, int64_t n)
+{
+    T->pts = pts;
+    T->n = n;
+    T->perm = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) T->perm[i] = (uint32_t)i;
+    T->cap_nodes = 1024;
+    T->n_nodes = 0;
+    T->nodes = (kdnode_t *)malloc(sizeof(kdnode_t) * (size_t)T->cap_nodes);
+    if (n > 0) kd_build_rec(T, 0, n);
+}
+
+static void kd_free(kdtree_t *T)
+{
+    free(T->perm);
+    free(T->nodes);
+}
+
+/* squared distance from q to a node's bounding box (double, exact lower bound) */
+static inline double box_dist2(const float *q, const kdnode_t *nd)
+{
+    double s = 0.0;
+    for (int c = 0; c < 3; ++c)
+    {
+        double t = 0.0;
+        if (q[c] < nd->bmin[c])
+            t = (double)nd->bmin[c] - (double)q[c];
+        else if (q[c] > nd->bmax[c])
+            t = (double)q[c] - (double)nd->bmax[c];
+        s += t * t;
+    }
+    return s;
+}
+
+/* Visit the tree; heap holds the best (d, idx) so far.  A node is pruned only
+   when its exact box distance exceeds the current worst by a margin larger
+   than the float rounding of the candidate distances. */
+static void kd_search(const kdtree_t *T, int32_t id, const float *q, bheap_t *h)
+{
+    const kdnode_t *nd = &T->nodes[id];
+    if (h->n == h->cap && box_dist2(q, nd) * (1.0 - 1e-6) > h->d[0]) return;
+    if (nd->left < 0)
+    {
+        for (int64_t j = nd->lo; j < nd->hi; ++j)
+        {
+            uint32_t pi = T->perm[j];
+            float d = sqdist_f32(q, T->pts + (size_t)pi * 3);
+            bheap_push(h, (double)d, pi);
+        }
+        return;
+    }
+    int32_t first = q[nd->dim] < nd->split ? nd->left : nd->right;
+    int32_t second = first == nd->left ? nd->right : nd->left;
+    kd_search(T, first, q, h);
+    kd_search(T, second, q, h);
+}
+
+typedef struct
+{
+    const kdtree_t *T;
+    const float *data;
+    int64_t n_data;
+    const float *query;
+    int64_t n_query;
+    int64_t cap;
+    int radius;
+    float r2;
+    uint32_t *idx_out; /* (n_query, cap) */
+    float *d_out;      /* (n_query, cap) */
+    int32_t *count_out;
+    int n_threads, tid;
+    int brute;
+} nn_job_t;
+
+static void *nn_worker(void *arg)
+{
+    nn_job_t *J = (nn_job_t *)arg;
+    double *hd = (double *)malloc(sizeof(double) * (size_t)(J->cap > 0 ? J->cap : 1));
+    uint32_t *hi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(J->cap > 0 ? J->cap : 1));
+    double *od = (double *)malloc(sizeof(double) * (size_t)(J->cap > 0 ? J->cap : 1));
+    uint32_t *oi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(J->cap > 0 ? J->cap : 1));
+    for (int64_t qi = J->tid; qi < J->n_query; qi += J->n_threads)
+    {
+        const float *q = J->query + (size_t)qi * 3;
+        bheap_t h = {hd, hi, J->cap, 0};
+        if (J->cap > 0)
+        {
+            if (J->radius)
+            {
+                /* RKNNResultSet: accepted iff d < r^2, best cap kept */
+                if (J->brute)
+                {
+                    for (int64_t j = 0; j < J->n_data; ++j)
+                    {
+                        float d = sqdist_f32(q, J->data + (size_t)j * 3);
+                        if (d < J->r2) bheap_push(&h, (double)d, (uint32_t)j);
+                    }
+                }
+                else
+                {
+                    /* seed the heap bound with r^2 by a pruning-only wrapper */
+                    const kdtree_t *T = J->T;
+                    /* explicit stack traversal with radius pruning */
+                    int32_t stack[256];
+                    int sp = 0;
+                    if (T->n_nodes > 0) stack[sp++] = 0;
+                    while (sp > 0)
+                    {
+                        const kdnode_t *nd = &T->nodes[stack[--sp]];
+                        double bd = box_dist2(q, nd) * (1.0 - 1e-6);
+                        if (bd > (double)J->r2) continue;
+                        if (h.n == h.cap && bd > h.d[0]) continue;
+                        if (nd->left < 0)
+                        {
+                            for (int64_t j = nd->lo; j < nd->hi; ++j)
+                            {
+                                uint32_t pi = T->perm[j];
+                                float d = sqdist_f32(q, T->pts + (size_t)pi * 3);
+                                if (d < J->r2) bheap_push(&h, (double)d, pi);
+                            }
+                        }
+                        else
+                        {
+                            stack[sp++] = nd->left;
+                            stack[sp++] = nd->right;
+                        }
+                    }
+                }
+            }
+            else
+            {
+                if (J->brute)
+                {
+                    for (int64_t j = 0; j < J->n_data; ++j)
+                        bheap_push(&h, (double)sqdist_f32(q, J->data + (size_t)j * 3), (uint32_t)j);
+                }
+                else
+                    kd_search(J->T, 0, q, &h);
+            }
+        }
+        int64_t n = h.n;
+        bheap_drain(&h, od, oi);
+        for (int64_t j = 0; j < J->cap; ++j)
+        {
+            if (j < n)
+            {
+                J->idx_out[(size_t)qi * J->cap + j] = oi[j];
+                J->d_out[(size_t)qi * J->cap + j] = (float)od[j];
+            }
+            else
+            {
+                J->idx_out[(size_t)qi * J->cap + j] = 0xFFFFFFFFu;
+                J->d_out[(size_t)qi * J->cap + j] = 0.0f;
+            }
+        }
+        /* RKNNResultSet::init leaves r^2 in the last slot of a row it does not fill */
+        if (J->radius && n < J->cap) J->d_out[(size_t)qi * J->cap + J->cap - 1] = J->r2;
+        if (J->count_out) J->count_out[qi] = (int32_t)n;
+    }
+    free(hd);
+    free(hi);
+    free(od);
+    free(oi);
+    return NULL;
+}
+
+static int run_nn(nn_job_t *base, int n_threads)
+{
+    if (n_threads < 1) n_threads = 1;
+    if (n_threads > 256) n_threads = 256;
+    pthread_t th[256];
+    nn_job_t jobs[256];
+    for (int t = 0; t < n_threads; ++t)
+    {
+        jobs[t] = *base;
+        jobs[t].n_threads = n_threads;
+        jobs[t].tid = t;
+    }
+    for (int t = 1; t < n_threads; ++t) pthread_create(&th[t], NULL, nn_worker, &jobs[t]);
+    nn_worker(&jobs[0]);
+    for (int t = 1; t < n_threads; ++t) pthread_join(th[t], NULL);
+    return 0;
+}
+
+/* nn_search.hpp:31-67.  idx/d: (n_query, knn). */
+int oracle_knn(const float *data, int64_t n_data, const float *query, int64_t n_query, uint32_t knn, int use_kdtree,
+               uint32_t *idx, float *d, int n_threads)
+{
+    if ((int64_t)knn > n_data) return 1; /* nn_search.hpp:37 */
+    nn_job_t J;
+    memset(&J, 0, sizeof J);
+    kdtree_t T;
+    memset(&T, 0, sizeof T);
+    if (use_kdtree) kd_build(&T, data, n_data);
+    J.T = &T, J.data = data, J.n_data = n_data, J.query = query, J.n_query = n_query, J.cap = knn;
+    J.idx_out = idx, J.d_out = d, J.brute = !use_kdtree;
+    run_nn(&J, n_threads);
+    if (use_kdtree) kd_free(&T);
+    return 0;
+}
+
+/* nn_search.hpp:85-132.  idx: int32 with -1 padding; d: 0 padding and r^2 in
+   the last column of rows with fewer than max_knn hits. */
+int oracle_radius(const float *data, int64_t n_data, const float *query, int64_t n_query, float radius,
+                  uint32_t max_knn, int use_kdtree, int32_t *idx, float *d, int32_t *counts, int n_threads)
+{
+    if ((int64_t)max_knn > n_data) return 1; /* nn_search.hpp:92-95 */
+    nn_job_t J;
+    memset(&J, 0, sizeof J);
+    kdtree_t T;
+    memset(&T, 0, sizeof T);
+    if (use_kdtree) kd_build(&T, data, n_data);
+    J.T = &T, J.data = data, J.n_data = n_data, J.query = query, J.n_query = n_query, J.cap = max_knn;
+    J.radius = 1, J.r2 = radius * radius;
+    J.idx_out = (uint32_t *)idx, J.d_out = d, J.count_out = counts, J.brute = !use_kdtree;
+    run_nn(&J, n_threads);
+    if (use_kdtree) kd_free(&T);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute_features_selected (pgeof.hpp:325-375 + pca.hpp:212-295).          */
+/* xyz float (is_f64 = 0) or double (is_f64 = 1).  Neighbourhood = the       */
+/* min(found, max_knn) nearest points with d^2 < r^2 (ties by index);        */
+/* rows with fewer than 2 hits stay 0.  out: (n, n_sel) doubles.             */
+/* ------------------------------------------------------------------------ */
+static double selected_value(const pca64_t *p, int fid, int is_f64)
+{
+    double s0 = sqrt(p->val[0]), s1 = sqrt(p->val[1]), s2 = sqrt(p->val[2]);
+    double fact = 1.0 / (s0 + ORACLE_EPS);
+    switch (fid)
+    {
+        case 0: return (s0 - s1) * fact;
+        case 1: return (s1 - s2) * fact;
+        case 2: return s2 * fact;
+        case 3:
+            if (s0 > 0.0)
+            {
+                double u[3];
+                for (int c = 0; c < 3; ++c)
+                    u[c] = p->val[0] * fabs(p->v[0][c]) + p->val[1] * fabs(p->v[1][c]) +
+                           p->val[2] * fabs(p->v[2][c]);
+                return u[2] / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            }
+            return 0.0;
+        case 4: return p->v[2][0];
+        case 5: return p->v[2][1];
+        case 6: return p->v[2][2];
+        case 7: return s0;
+        /* pca.hpp:252 adds the float literal 1e-6f in both instantiations */
+        case 8: return sqrt(s0 * s1 + (is_f64 ? (double)1e-6f : 1e-6));
+        case 9: return pow(s0 * s1 * s2 + 1e-9, 1.0 / 3.0);
+        case 10: return s2 / (s0 + s1 + s2 + ORACLE_EPS);
+        case 12: return 1.0 - fabs(p->v[2][2]);
+        case 13: return eigentropy64(p->val);
+        default: return 0.0;
+    }
+}
+
+int oracle_compute_features_selected(const void *xyz, int is_f64, int64_t n, double radius, uint32_t max_knn,
+                                     const int32_t *fids, int64_t n_sel, double *out)
+{
+    memset(out, 0, sizeof(double) * (size_t)n * (size_t)(n_sel > 0 ? n_sel : 0));
+    double *hd = (double *)malloc(sizeof(double) * (size_t)(max_knn + 1));
+    uint32_t *hi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(max_knn + 1));
+    double *od = (double *)malloc(sizeof(double) * (size_t)(max_knn + 1));
+    uint32_t *oi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(max_knn + 1));
+    const float *xf = (const float *)xyz;
+    const double *xd = (const double *)xyz;
+    float r2f = (float)radius * (float)radius;
+    double r2d = radius * radius;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        bheap_t h = {hd, hi, max_knn, 0};
+        int64_t found = 0;
+        for (int64_t j = 0; j < n; ++j)
+        {
+            int in;
+            double d;
+            if (is_f64)
+            {
+                d = sqdist_f64(xd + (size_t)i * 3, xd + (size_t)j * 3);
+                in = d < r2d;
+            }
+            else
+            {
+                float df = sqdist_f32(xf + (size_t)i * 3, xf + (size_t)j * 3);
+                d = (double)df;
+                in = df < r2f;
+            }
+            if (in)
+            {
+                found++;
+                if (max_knn > 0) bheap_push(&h, d, (uint32_t)j);
+            }
+        }
+        if (found < 2) continue;
+        int64_t m = h.n;
+        if (m < 1) continue;
+        bheap_drain(&h, od, oi);
+        double cov[6];
+        if (is_f64)
+            cov_two_pass_f64(xd, oi, (size_t)m, cov);
+        else
+            cov_two_pass(xf, oi, (size_t)m, cov);
+        pca64_t p;
+        pca_from_cov(cov, &p);
+        for (int64_t s = 0; s < n_sel; ++s) out[(size_t)i * (size_t)n_sel + (size_t)s] = selected_value(&p, fids[s], is_f64);
+    }
+    free(hd);
+    free(hi);
+    free(od);
+    free(oi);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPU baseline pipeline: KD-tree kNN for a sample of queries of the full    */
+/* cloud, then the FP64 features of those neighbourhoods.  Timings in        */
+/* seconds are returned for the build and for the query+feature part.       */
+/* ------------------------------------------------------------------------ */
+#include <time.h>
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int oracle_knn_features_timed(const float *xyz, int64_t n, const uint32_t *sample, int64_t n_sample, uint32_t k,
+                              int n_threads, double *out_features, uint32_t *out_idx, double *t_build,
+                              double *t_query)
+{
+    if ((int64_t)k > n) return 1;
+    double t0 = now_s();
+    kdtree_t T;
+    kd_build(&T, xyz, n);
+    double t1 = now_s();
+    float *q = (float *)malloc(sizeof(float) * 3 * (size_t)(n_sample > 0 ? n_sample : 1));
+    for (int64_t i = 0; i < n_sample; ++i)
+        for (int c = 0; c < 3; ++c) q[i * 3 + c] = xyz[(size_t)sample[i] * 3 + c];
+    float *dd = (float *)malloc(sizeof(float) * (size_t)k * (size_t)(n_sample > 0 ? n_sample : 1));
+    nn_job_t J;
+    memset(&J, 0, sizeof J);
+    J.T = &T, J.data = xyz, J.n_data = n, J.query = q, J.n_query = n_sample, J.cap = k;
+    J.idx_out = out_idx, J.d_out = dd;
+    run_nn(&J, n_threads);
+    uint32_t *ptr = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n_sample + 1));
+    for (int64_t i = 0; i <= n_sample; ++i) ptr[i] = (uint32_t)(i * k);
+    csr_job_t C;
+    memset(&C, 0, sizeof C);
+    C.xyz = xyz, C.n_xyz = n, C.nn = out_idx, C.n_nn = n_sample * k, C.nn_ptr = ptr, C.n_points = n_sample;
+    C.k_min = 1, C.out = out_features, C.mode = 0;
+    memset(out_features, 0, sizeof(double) * 11 * (size_t)n_sample);
+    run_csr(&C, n_threads);
+    double t2 = now_s();
+    *t_build = t1 - t0;
+    *t_query = t2 - t1;
+    free(ptr);
+    free(dd);
+    free(q);
+    kd_free(&T);
+    return 0;
+}

@@ -1,0 +1,982 @@
+// Host runtime and C ABI (include/pgeof_hip.h) of the gfx950 pgeof path.
+//
+// Responsibilities:
+//   - argument validation with the reference's error conditions and messages
+//     (include/pgeof.hpp:81, 165-168, 250; include/nn_search.hpp:37, 92-95)
+//   - grid sizing for the neighbour search (bounding box read back, one
+//     occupancy probe for kNN)
+//   - workspace carving, host<->device staging for the host-memory API
+//   - optional per-kernel HIP-event profiling
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pgeof_hip.h"
+#include "pgeof_internal.hpp"
+
+using namespace pgeof_amd;
+
+namespace
+{
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string& msg)
+{
+    t_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where)
+{
+    return fail(e == hipErrorOutOfMemory ? PGEOF_ENOMEM : PGEOF_ERUNTIME,
+                std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+    do                                                  \
+    {                                                   \
+        hipError_t _e = (expr);                         \
+        if (_e != hipSuccess) return hip_fail(_e, #expr); \
+    } while (0)
+
+// ------------------------------------------------------------- tuning ----
+struct Tuning
+{
+    double knn_occupancy = 0.08;       // target points per cell = max(0.75, this * k)
+    double grid_cells_per_point = 4.0;  // cap on G / n
+    double lds_budget = 65536.0;        // bytes of LDS per workgroup for heaps
+    double radius_cell_factor = 0.5;    // cell edge = this * radius
+};
+Tuning g_tune;
+std::mutex g_tune_mu;
+
+Tuning tuning()
+{
+    std::lock_guard<std::mutex> l(g_tune_mu);
+    return g_tune;
+}
+
+// ---------------------------------------------------------- profiling ----
+std::atomic<int> g_prof{0};
+struct ProfRec
+{
+    std::string name;
+    hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+std::vector<ProfRec> g_prof_pending;
+std::map<std::string, std::pair<int64_t, double>> g_prof_acc;
+
+// ------------------------------------------------------- device state ----
+struct DevState
+{
+    int lds_max = 65536;
+    bool init = false;
+};
+std::mutex g_dev_mu;
+std::map<int, DevState> g_dev;
+
+int lds_limit()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    std::lock_guard<std::mutex> l(g_dev_mu);
+    DevState& d = g_dev[dev];
+    if (!d.init)
+    {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            d.lds_max = v;
+        d.init = true;
+    }
+    return d.lds_max;
+}
+
+// Per-thread host-API context: a stream, a grow-only device buffer and a
+// small pinned host buffer for read-backs.
+struct ThreadCtx
+{
+    int dev = -1;
+    hipStream_t stream = nullptr;
+    char* dbuf = nullptr;
+    size_t dbytes = 0;
+    void* pinned = nullptr;
+    ~ThreadCtx()
+    {
+        // device resources are left to process teardown
+    }
+};
+thread_local ThreadCtx t_ctx;
+
+int ctx_prepare(hipStream_t* s)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (t_ctx.dev != dev)
+    {
+        t_ctx.dev = dev;
+        t_ctx.stream = nullptr;
+        t_ctx.dbuf = nullptr;
+        t_ctx.dbytes = 0;
+        t_ctx.pinned = nullptr;
+    }
+    if (!t_ctx.stream) HIP_TRY(hipStreamCreateWithFlags(&t_ctx.stream, hipStreamNonBlocking));
+    *s = t_ctx.stream;
+    return PGEOF_OK;
+}
+
+int ctx_reserve(size_t bytes)
+{
+    if (bytes <= t_ctx.dbytes) return PGEOF_OK;
+    if (t_ctx.dbuf)
+    {
+        (void)hipStreamSynchronize(t_ctx.stream);
+        (void)hipFree(t_ctx.dbuf);
+        t_ctx.dbuf = nullptr;
+        t_ctx.dbytes = 0;
+    }
+    size_t want = bytes + bytes / 8 + (1 << 20);
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, want));
+    t_ctx.dbuf = (char*)p;
+    t_ctx.dbytes = want;
+    return PGEOF_OK;
+}
+
+void* pinned_scratch()
+{
+    if (!t_ctx.pinned)
+    {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, 4096, hipHostMallocDefault) != hipSuccess) return nullptr;
+        t_ctx.pinned = p;
+    }
+    return t_ctx.pinned;
+}
+
+inline size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+
+// ------------------------------------------------------------- grids ----
+uint64_t grid_cap(uint64_t n)
+{
+    const Tuning t = tuning();
+    double c = std::max(1.0, t.grid_cells_per_point) * (double)n;
+    c = std::max(c, 65536.0);
+    c = std::min(c, 2147483000.0);
+    return (uint64_t)c;
+}
+
+uint64_t cells_for(const double ext[3], double h, int g[3])
+{
+    uint64_t G = 1;
+    for (int a = 0; a < 3; ++a)
+    {
+        double c = std::floor(ext[a] / h) + 1.0;
+        if (!(c >= 1.0)) c = 1.0;
+        if (c > 1048576.0) c = 1048576.0;
+        g[a] = (int)c;
+        G *= (uint64_t)g[a];
+    }
+    return G;
+}
+
+// smallest h (up to 0.1%) with cells_for(h) <= cap
+double h_for_cap(const double ext[3], uint64_t cap)
+{
+    const double emax = std::max(ext[0], std::max(ext[1], ext[2]));
+    if (!(emax > 0.0)) return 1.0;
+    int g[3];
+    double hi = emax * 2.0, lo = emax * 1e-7;
+    if (cells_for(ext, lo, g) <= cap) return lo;
+    for (int it = 0; it < 80; ++it)
+    {
+        const double mid = std::sqrt(lo * hi);
+        if (cells_for(ext, mid, g) <= cap)
+            hi = mid;
+        else
+            lo = mid;
+        if (hi / lo < 1.001) break;
+    }
+    return hi;
+}
+
+GridParams make_grid(const float bb[6], double h)
+{
+    GridParams gp;
+    double ext[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        gp.lo[a] = (double)bb[a];
+        ext[a] = std::max(0.0, (double)bb[3 + a] - (double)bb[a]);
+    }
+    gp.h = h;
+    gp.inv_h = 1.0 / h;
+    gp.G = cells_for(ext, h, gp.g);
+    return gp;
+}
+
+// Layout of a search workspace.
+struct SearchWs
+{
+    float* bbox_partial;
+    float* bbox_out;
+    unsigned long long* sumsq;
+    uint32_t* count;
+    uint32_t* cell_start;
+    uint32_t* scan_part;
+    uint32_t* cell;
+    uint32_t* rank;
+    float4* pts;
+    uint32_t* qcell;
+    uint32_t* qrank;
+    float4* qs;
+    uint64_t* gheap_keys;
+    uint32_t* gheap_pos;
+    uint32_t* dense;   // radius csr: n_query * cap
+    uint32_t* counts;  // radius csr: n_query (+1 for the scan)
+    uint32_t* csr_part;
+};
+
+constexpr uint32_t kBboxBlocks = 1024;
+
+struct HeapPlan
+{
+    int block;
+    size_t lds;
+    bool global;
+};
+
+HeapPlan heap_plan(uint32_t k, bool with_pos)
+{
+    const Tuning t = tuning();
+    const size_t per = (size_t)k * (8 + (with_pos ? 4 : 0));
+    const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
+    for (int b : {256, 128, 64})
+        if (per * (size_t)b <= budget) return {b, per * (size_t)b, false};
+    return {256, 0, true};
+}
+
+size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool with_pos, bool csr)
+{
+    const uint64_t G = grid_cap(n) + 1;
+    size_t b = 0;
+    b += al(kBboxBlocks * 6 * sizeof(float)) + al(64) + al(64);
+    b += 2 * al(G * 4) + al(scan_partials_needed(G) * 4);
+    b += 2 * al(n * 4) + al(n * 16);
+    b += 2 * al(m * 4) + al(m * 16);
+    const HeapPlan hp = heap_plan(k, with_pos);
+    if (hp.global) b += al(m * k * 8) + al(m * k * 4);
+    if (csr) b += al(m * k * 4) + al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4);
+    return b + 4096;
+}
+
+SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool with_pos, bool csr)
+{
+    SearchWs w{};
+    const uint64_t G = grid_cap(n) + 1;
+    w.bbox_partial = ar.take<float>(kBboxBlocks * 6);
+    w.bbox_out = ar.take<float>(16);
+    w.sumsq = ar.take<unsigned long long>(8);
+    w.count = ar.take<uint32_t>(G);
+    w.cell_start = ar.take<uint32_t>(G);
+    w.scan_part = ar.take<uint32_t>(scan_partials_needed(G));
+    w.cell = ar.take<uint32_t>(n);
+    w.rank = ar.take<uint32_t>(n);
+    w.pts = ar.take<float4>(n);
+    w.qcell = ar.take<uint32_t>(m);
+    w.qrank = ar.take<uint32_t>(m);
+    w.qs = ar.take<float4>(m);
+    const HeapPlan hp = heap_plan(k, with_pos);
+    if (hp.global)
+    {
+        w.gheap_keys = ar.take<uint64_t>(m * k);
+        w.gheap_pos = ar.take<uint32_t>(m * k);
+    }
+    if (csr)
+    {
+        w.dense = ar.take<uint32_t>(m * k);
+        w.counts = ar.take<uint32_t>(m + 1);
+        w.csr_part = ar.take<uint32_t>(scan_partials_needed(m + 1));
+    }
+    return w;
+}
+
+// Read `bytes` from device to host through pinned memory (synchronises s).
+int readback(void* host, const void* dev, size_t bytes, hipStream_t s)
+{
+    void* pin = pinned_scratch();
+    if (!pin || bytes > 4096) return fail(PGEOF_ERUNTIME, "pinned staging buffer unavailable");
+    HIP_TRY(hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::memcpy(host, pin, bytes);
+    return PGEOF_OK;
+}
+
+// Build the sorted grid of `data` and (when query != data) sort the queries
+// into the same grid.  mode: kModeKnn/kModeKnnFeat (k used for sizing) or a
+// radius mode (radius used for sizing).
+int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, uint32_t k, float radius, bool is_radius,
+               SearchWs& w, GridView& gv, const float4** qs_out, hipStream_t s)
+{
+    const uint32_t nbb = std::max<uint32_t>(1, std::min<uint32_t>(kBboxBlocks, (n + 255) / 256));
+    HIP_TRY(launch_bbox(data, n, w.bbox_partial, nbb, w.bbox_out, s));
+    float bb[6];
+    int rc = readback(bb, w.bbox_out, sizeof bb, s);
+    if (rc) return rc;
+    bool finite = true;
+    for (int i = 0; i < 3; ++i)
+        if (!(bb[i] <= bb[3 + i])) finite = false;
+    if (!finite)
+        for (int i = 0; i < 3; ++i) bb[i] = 0.0f, bb[3 + i] = 0.0f;
+    double ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = std::max(0.0, (double)bb[3 + a] - (double)bb[a]);
+    const double emax = std::max(ext[0], std::max(ext[1], ext[2]));
+    const uint64_t cap = grid_cap(n);
+    const double hcap = h_for_cap(ext, cap);
+    double h;
+    const Tuning t = tuning();
+    if (!(emax > 0.0))
+        h = 1.0;
+    else if (is_radius)
+    {
+        double r = std::fabs((double)radius);
+        h = std::isfinite(r) && r > 0.0 ? r * t.radius_cell_factor : emax;
+        h = std::max(h, hcap);
+    }
+    else
+    {
+        // target occupancy T; first guess assumes a uniformly filled box,
+        // corrected by the measured point-weighted occupancy assuming the
+        // points lie on 2-D surfaces (LiDAR); clamped by the grid-size cap.
+        const double T = std::max(0.75, t.knn_occupancy * (double)k);
+        double vol = 1.0;
+        for (int a = 0; a < 3; ++a) vol *= std::max(ext[a], emax * 1e-3);
+        double h1 = std::cbrt(vol * T / std::max<double>(n, 1.0));
+        h1 = std::max(h1, hcap);
+        GridParams g1 = make_grid(bb, h1);
+        HIP_TRY(launch_fill_u32(w.count, g1.G, 0u, s));
+        HIP_TRY(launch_hist(data, n, g1, w.count, nullptr, nullptr, s));
+        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sumsq), 2, 0u, s));
+        HIP_TRY(launch_sumsq(w.count, g1.G, w.sumsq, s));
+        unsigned long long ss = 0;
+        rc = readback(&ss, w.sumsq, sizeof ss, s);
+        if (rc) return rc;
+        const double m1 = (double)ss / std::max<double>(n, 1.0);
+        h = h1;
+        if (m1 > T) h = h1 * std::sqrt(T / m1);
+        h = std::max(h, hcap);
+    }
+    gv.p = make_grid(bb, h);
+    const uint64_t G = gv.p.G;
+    HIP_TRY(launch_fill_u32(w.count, G + 1, 0u, s));
+    HIP_TRY(launch_hist(data, n, gv.p, w.count, w.cell, w.rank, s));
+    HIP_TRY(launch_scan_exclusive(w.count, w.cell_start, G + 1, w.scan_part, s));
+    HIP_TRY(launch_scatter(data, n, w.cell, w.rank, w.cell_start, w.pts, s));
+    gv.cell_start = w.cell_start;
+    gv.pts = w.pts;
+    if (query == data && m == n)
+        *qs_out = w.pts;
+    else
+    {
+        HIP_TRY(launch_fill_u32(w.count, G + 1, 0u, s));
+        HIP_TRY(launch_hist(query, m, gv.p, w.count, w.qcell, w.qrank, s));
+        HIP_TRY(launch_scan_exclusive(w.count, w.count, G + 1, w.scan_part, s));
+        HIP_TRY(launch_scatter(query, m, w.qcell, w.qrank, w.count, w.qs, s));
+        *qs_out = w.qs;
+    }
+    return PGEOF_OK;
+}
+
+SearchArgs base_args(const GridView& gv, const float4* qs, uint32_t m, uint32_t k)
+{
+    SearchArgs a{};
+    a.grid = gv;
+    a.queries = qs;
+    a.n_query = m;
+    a.k = k;
+    a.margin = (float)(gv.p.h * 1e-6);
+    return a;
+}
+
+// Device search core shared by the host and device entry points.
+int run_search(int mode, const float* data, uint32_t n, const float* query, uint32_t m, uint32_t k, float radius,
+               void* out_idx, float* out_d2, float* out_feat, uint32_t* out_count, Arena& ar, hipStream_t s,
+               SearchWs* ws_out = nullptr)
+{
+    const bool with_pos = mode == kModeKnnFeat;
+    const bool csr = mode == kModeRadiusCsr;
+    SearchWs w = carve(ar, n, m, k, with_pos, csr);
+    if (ar.off > ar.size) return fail(PGEOF_ENOMEM, "workspace too small");
+    GridView gv;
+    const float4* qs = nullptr;
+    const bool is_radius = mode == kModeRadius || mode == kModeRadiusCsr;
+    int rc = build_grid(data, n, query, m, k, radius, is_radius, w, gv, &qs, s);
+    if (rc) return rc;
+    SearchArgs a = base_args(gv, qs, m, k);
+    a.r2 = radius * radius;
+    a.out_idx = csr ? (void*)w.dense : out_idx;
+    a.out_d2 = out_d2;
+    a.out_feat = out_feat;
+    a.out_count = csr ? w.counts : out_count;
+    const HeapPlan hp = heap_plan(k, with_pos);
+    a.gheap_keys = w.gheap_keys;
+    a.gheap_pos = w.gheap_pos;
+    HIP_TRY(launch_search(mode, a, hp.block, hp.lds, hp.global, s));
+    if (ws_out) *ws_out = w;
+    return PGEOF_OK;
+}
+
+bool ok_size(int64_t v) { return v >= 0 && v < (int64_t)0xFFFFFFFFLL; }
+
+int validate_knn(int64_t n_data, int64_t n_query, uint64_t knn, bool radius)
+{
+    if (!ok_size(n_data) || !ok_size(n_query)) return fail(PGEOF_EINVAL, "point cloud too large for uint32 indices");
+    if ((int64_t)knn > n_data)
+        return fail(PGEOF_EINVAL, radius ? "max knn size is greater than the data point cloud size"
+                                         : "knn size is greater than the data point cloud size");
+    if ((uint64_t)n_query * knn >= (1ull << 40)) return fail(PGEOF_EINVAL, "output too large");
+    return PGEOF_OK;
+}
+
+int validate_csr(int64_t n_xyz, int64_t n_nn, int64_t n_nn_ptr)
+{
+    if (n_nn_ptr < 1) return fail(PGEOF_EINVAL, "nn_ptr must hold at least one entry");
+    if (!ok_size(n_xyz) || !ok_size(n_nn_ptr - 1) || n_nn < 0) return fail(PGEOF_EINVAL, "input too large");
+    return PGEOF_OK;
+}
+
+bool check_scales(const uint32_t* s, int64_t n)
+{
+    uint32_t prev = 1;  // pgeof.hpp:123-132
+    for (int64_t i = 0; i < n; ++i)
+    {
+        if (s[i] < prev) return false;
+        prev = s[i];
+    }
+    return true;
+}
+
+int err_from_word(uint32_t w)
+{
+    if (w & kErrNnPtr) return fail(PGEOF_EINVAL, "nn_ptr is not non-decreasing or points past the end of nn");
+    if (w & kErrNnIndex) return fail(PGEOF_EINVAL, "nn holds an index >= the number of xyz points");
+    return PGEOF_OK;
+}
+
+// Device CSR driver: modes 0 plain, 1 multiscale, 2 optimal.
+int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn, const uint32_t* nn_ptr,
+               int64_t n_nn_ptr, uint32_t k_min, uint32_t k_step, uint32_t k_min_search, const uint32_t* d_scales,
+               uint32_t n_scales, float* out, uint32_t* err, hipStream_t s)
+{
+    CsrArgs a{};
+    a.xyz = xyz;
+    a.n_xyz = (uint32_t)n_xyz;
+    a.nn = nn;
+    a.n_nn = (uint64_t)n_nn;
+    a.nn_ptr = nn_ptr;
+    a.n_points = (uint32_t)(n_nn_ptr - 1);
+    a.k_min = k_min;
+    a.k_step = k_step;
+    a.k_min_search = k_min_search;
+    a.scales = d_scales;
+    a.n_scales = n_scales;
+    a.out = out;
+    a.err = err;
+    HIP_TRY(launch_csr_features(mode, a, s));
+    return PGEOF_OK;
+}
+
+// Host-memory CSR driver.
+int csr_host(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn, const uint32_t* nn_ptr,
+             int64_t n_nn_ptr, uint32_t k_min, uint32_t k_step, uint32_t k_min_search, const uint32_t* scales,
+             int64_t n_scales, float* out)
+{
+    const uint64_t n_points = (uint64_t)(n_nn_ptr - 1);
+    const uint64_t n_out = n_points * (mode == 1 ? (uint64_t)n_scales * 11 : (mode == 2 ? 12 : 11));
+    if (n_points == 0) return PGEOF_OK;
+    hipStream_t s;
+    int rc = ctx_prepare(&s);
+    if (rc) return rc;
+    const size_t b_xyz = al((size_t)n_xyz * 12 + 4), b_nn = al((size_t)n_nn * 4 + 4),
+                 b_ptr = al((size_t)n_nn_ptr * 4), b_sc = al((size_t)n_scales * 4 + 4), b_out = al(n_out * 4),
+                 b_err = al(4);
+    rc = ctx_reserve(b_xyz + b_nn + b_ptr + b_sc + b_out + b_err);
+    if (rc) return rc;
+    Arena ar{t_ctx.dbuf, t_ctx.dbytes, 0};
+    float* d_xyz = ar.take<float>((size_t)n_xyz * 3 + 1);
+    uint32_t* d_nn = ar.take<uint32_t>((size_t)n_nn + 1);
+    uint32_t* d_ptr = ar.take<uint32_t>((size_t)n_nn_ptr);
+    uint32_t* d_sc = ar.take<uint32_t>((size_t)n_scales + 1);
+    float* d_out = ar.take<float>(n_out);
+    uint32_t* d_err = ar.take<uint32_t>(1);
+    if (n_xyz) HIP_TRY(hipMemcpyAsync(d_xyz, xyz, (size_t)n_xyz * 12, hipMemcpyHostToDevice, s));
+    if (n_nn) HIP_TRY(hipMemcpyAsync(d_nn, nn, (size_t)n_nn * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_ptr, nn_ptr, (size_t)n_nn_ptr * 4, hipMemcpyHostToDevice, s));
+    if (n_scales) HIP_TRY(hipMemcpyAsync(d_sc, scales, (size_t)n_scales * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_fill_u32(d_err, 1, 0u, s));
+    rc = csr_device(mode, d_xyz, n_xyz, d_nn, n_nn, d_ptr, n_nn_ptr, k_min, k_step, k_min_search, d_sc,
+                    (uint32_t)n_scales, d_out, d_err, s);
+    if (rc) return rc;
+    uint32_t werr = 0;
+    HIP_TRY(hipMemcpyAsync(out, d_out, n_out * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&werr, d_err, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return err_from_word(werr);
+}
+
+// Device-memory wrapper: workspace from the caller or stream-ordered.
+struct WsGuard
+{
+    void* p = nullptr;
+    hipStream_t s;
+    bool owned = false;
+    ~WsGuard()
+    {
+        if (owned && p) (void)hipFreeAsync(p, s);
+    }
+};
+
+int acquire_ws(WsGuard& g, void* ws, size_t ws_bytes, size_t need, hipStream_t s, Arena& ar)
+{
+    g.s = s;
+    if (ws)
+    {
+        if (ws_bytes < need) return fail(PGEOF_EINVAL, "workspace smaller than the size returned by *_workspace_size");
+        ar = Arena{(char*)ws, ws_bytes, 0};
+        return PGEOF_OK;
+    }
+    HIP_TRY(hipMallocAsync(&g.p, need, s));
+    g.owned = true;
+    ar = Arena{(char*)g.p, need, 0};
+    return PGEOF_OK;
+}
+
+}  // namespace
+
+// ===================================================== profiling hooks ====
+namespace pgeof_amd
+{
+void prof_begin(const char* name, hipStream_t s, void** tok)
+{
+    *tok = nullptr;
+    if (!g_prof.load(std::memory_order_relaxed)) return;
+    ProfRec* r = new ProfRec{name, nullptr, nullptr};
+    if (hipEventCreate(&r->a) != hipSuccess || hipEventCreate(&r->b) != hipSuccess)
+    {
+        delete r;
+        return;
+    }
+    (void)hipEventRecord(r->a, s);
+    *tok = r;
+}
+
+void prof_end(void* tok, hipStream_t s)
+{
+    if (!tok) return;
+    ProfRec* r = (ProfRec*)tok;
+    (void)hipEventRecord(r->b, s);
+    std::lock_guard<std::mutex> l(g_prof_mu);
+    g_prof_pending.push_back(*r);
+    delete r;
+}
+}  // namespace pgeof_amd
+
+// ============================================================== C ABI =====
+extern "C" {
+
+const char* pgeof_last_error(void) { return t_err.c_str(); }
+
+int pgeof_abi_version(void) { return 100; }
+
+int pgeof_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pgeof_set_tuning(const char* key, double v)
+{
+    std::lock_guard<std::mutex> l(g_tune_mu);
+    const std::string k = key ? key : "";
+    if (!(v > 0.0) || !std::isfinite(v)) return fail(PGEOF_EINVAL, "tuning value must be positive");
+    if (k == "knn_occupancy")
+        g_tune.knn_occupancy = v;
+    else if (k == "grid_cells_per_point")
+        g_tune.grid_cells_per_point = v;
+    else if (k == "lds_budget")
+        g_tune.lds_budget = v;
+    else if (k == "radius_cell_factor")
+        g_tune.radius_cell_factor = v;
+    else
+        return fail(PGEOF_EINVAL, "unknown tuning key " + k);
+    return PGEOF_OK;
+}
+
+double pgeof_get_tuning(const char* key)
+{
+    std::lock_guard<std::mutex> l(g_tune_mu);
+    const std::string k = key ? key : "";
+    if (k == "knn_occupancy") return g_tune.knn_occupancy;
+    if (k == "grid_cells_per_point") return g_tune.grid_cells_per_point;
+    if (k == "lds_budget") return g_tune.lds_budget;
+    if (k == "radius_cell_factor") return g_tune.radius_cell_factor;
+    return -1.0;
+}
+
+void pgeof_profile_enable(int on) { g_prof.store(on ? 1 : 0); }
+
+void pgeof_profile_reset(void)
+{
+    std::lock_guard<std::mutex> l(g_prof_mu);
+    for (auto& r : g_prof_pending)
+    {
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_pending.clear();
+    g_prof_acc.clear();
+}
+
+int pgeof_profile_report(char* buf, size_t len)
+{
+    std::lock_guard<std::mutex> l(g_prof_mu);
+    for (auto& r : g_prof_pending)
+    {
+        (void)hipEventSynchronize(r.b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        auto& acc = g_prof_acc[r.name];
+        acc.first += 1;
+        acc.second += ms;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_pending.clear();
+    std::string js = "{";
+    bool first = true;
+    for (auto& kv : g_prof_acc)
+    {
+        char tmp[256];
+        std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%lld, %.6f]", first ? "" : ", ", kv.first.c_str(),
+                      (long long)kv.second.first, kv.second.second);
+        js += tmp;
+        first = false;
+    }
+    js += "}";
+    if (buf && len)
+    {
+        std::strncpy(buf, js.c_str(), len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int)js.size();
+}
+
+// ------------------------------------------------------ host searches ----
+int pgeof_knn_search(const float* data, int64_t n_data, const float* query, int64_t n_query, uint32_t knn,
+                     uint32_t* indices, float* sqr_dist)
+{
+    int rc = validate_knn(n_data, n_query, knn, false);
+    if (rc) return rc;
+    if (n_query == 0 || knn == 0) return PGEOF_OK;
+    hipStream_t s;
+    rc = ctx_prepare(&s);
+    if (rc) return rc;
+    const bool self = (query == data && n_query == n_data);
+    const size_t b_d = al((size_t)n_data * 12), b_q = self ? 0 : al((size_t)n_query * 12),
+                 b_i = al((size_t)n_query * knn * 4), b_o = al((size_t)n_query * knn * 4);
+    const size_t b_ws = search_ws_bytes((uint64_t)n_data, (uint64_t)n_query, knn, false, false);
+    rc = ctx_reserve(b_d + b_q + b_i + b_o + b_ws);
+    if (rc) return rc;
+    Arena ar{t_ctx.dbuf, t_ctx.dbytes, 0};
+    float* d_data = ar.take<float>((size_t)n_data * 3);
+    float* d_query = self ? d_data : ar.take<float>((size_t)n_query * 3);
+    uint32_t* d_idx = ar.take<uint32_t>((size_t)n_query * knn);
+    float* d_d2 = ar.take<float>((size_t)n_query * knn);
+    HIP_TRY(hipMemcpyAsync(d_data, data, (size_t)n_data * 12, hipMemcpyHostToDevice, s));
+    if (!self) HIP_TRY(hipMemcpyAsync(d_query, query, (size_t)n_query * 12, hipMemcpyHostToDevice, s));
+    Arena wa{t_ctx.dbuf + ar.off, t_ctx.dbytes - ar.off, 0};
+    rc = run_search(kModeKnn, d_data, (uint32_t)n_data, d_query, (uint32_t)n_query, knn, 0.0f, d_idx, d_d2, nullptr,
+                    nullptr, wa, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(indices, d_idx, (size_t)n_query * knn * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(sqr_dist, d_d2, (size_t)n_query * knn * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return PGEOF_OK;
+}
+
+int pgeof_radius_search(const float* data, int64_t n_data, const float* query, int64_t n_query,
+                        float search_radius, uint32_t max_knn, int32_t* indices, float* sqr_dist)
+{
+    int rc = validate_knn(n_data, n_query, max_knn, true);
+    if (rc) return rc;
+    if (n_query == 0 || max_knn == 0) return PGEOF_OK;
+    hipStream_t s;
+    rc = ctx_prepare(&s);
+    if (rc) return rc;
+    const bool self = (query == data && n_query == n_data);
+    const size_t b_d = al((size_t)n_data * 12), b_q = self ? 0 : al((size_t)n_query * 12),
+                 b_i = al((size_t)n_query * max_knn * 4), b_o = al((size_t)n_query * max_knn * 4);
+    const size_t b_ws = search_ws_bytes((uint64_t)n_data, (uint64_t)n_query, max_knn, false, false);
+    rc = ctx_reserve(b_d + b_q + b_i + b_o + b_ws);
+    if (rc) return rc;
+    Arena ar{t_ctx.dbuf, t_ctx.dbytes, 0};
+    float* d_data = ar.take<float>((size_t)n_data * 3);
+    float* d_query = self ? d_data : ar.take<float>((size_t)n_query * 3);
+    int32_t* d_idx = ar.take<int32_t>((size_t)n_query * max_knn);
+    float* d_d2 = ar.take<float>((size_t)n_query * max_knn);
+    HIP_TRY(hipMemcpyAsync(d_data, data, (size_t)n_data * 12, hipMemcpyHostToDevice, s));
+    if (!self) HIP_TRY(hipMemcpyAsync(d_query, query, (size_t)n_query * 12, hipMemcpyHostToDevice, s));
+    Arena wa{t_ctx.dbuf + ar.off, t_ctx.dbytes - ar.off, 0};
+    rc = run_search(kModeRadius, d_data, (uint32_t)n_data, d_query, (uint32_t)n_query, max_knn, search_radius, d_idx,
+                    d_d2, nullptr, nullptr, wa, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(indices, d_idx, (size_t)n_query * max_knn * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(sqr_dist, d_d2, (size_t)n_query * max_knn * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return PGEOF_OK;
+}
+
+// ----------------------------------------------------- host features ----
+int pgeof_compute_features(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                           const uint32_t* nn_ptr, int64_t n_nn_ptr, uint64_t k_min, float* features)
+{
+    if (k_min < 1) return fail(PGEOF_EINVAL, "k_min should be > 1");
+    int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
+    if (rc) return rc;
+    const uint32_t km = (uint32_t)std::min<uint64_t>(k_min, 0xFFFFFFFFull);
+    return csr_host(0, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, km, 1, 1, nullptr, 0, features);
+}
+
+int pgeof_compute_features_multiscale(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                      const uint32_t* nn_ptr, int64_t n_nn_ptr, const uint32_t* k_scales,
+                                      int64_t n_scales, float* features)
+{
+    if (n_scales < 0 || !check_scales(k_scales, n_scales))
+        return fail(PGEOF_EINVAL, "k_scales should be > 1 and sorted in ascending order");
+    int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
+    if (rc) return rc;
+    if (n_scales == 0) return PGEOF_OK;
+    return csr_host(1, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, 1, 1, 1, k_scales, n_scales, features);
+}
+
+int pgeof_compute_features_optimal(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                   const uint32_t* nn_ptr, int64_t n_nn_ptr, uint32_t k_min, uint32_t k_step,
+                                   uint32_t k_min_search, float* features)
+{
+    if (k_min < 1 && k_min_search < 1) return fail(PGEOF_EINVAL, "k_min and k_min_search should be > 1");
+    if (k_step < 1) return fail(PGEOF_EINVAL, "k_step should be >= 1");
+    int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
+    if (rc) return rc;
+    return csr_host(2, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, k_min, k_step, k_min_search, nullptr, 0, features);
+}
+
+int pgeof_compute_features_selected_f32(const float* xyz, int64_t n, float search_radius, uint32_t max_knn,
+                                        const int32_t* selected, int64_t n_selected, float* features)
+{
+    return fail(PGEOF_ERUNTIME, "compute_features_selected: not available in this build");
+}
+
+int pgeof_compute_features_selected_f64(const double* xyz, int64_t n, double search_radius, uint32_t max_knn,
+                                        const int32_t* selected, int64_t n_selected, double* features)
+{
+    return fail(PGEOF_ERUNTIME, "compute_features_selected: not available in this build");
+}
+
+// ---------------------------------------------------- device searches ----
+size_t pgeof_knn_workspace_size(int64_t n_data, int64_t n_query, uint32_t knn)
+{
+    return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), knn, true,
+                           false);
+}
+
+size_t pgeof_radius_workspace_size(int64_t n_data, int64_t n_query, uint32_t max_knn)
+{
+    return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), max_knn,
+                           false, false);
+}
+
+size_t pgeof_radius_csr_workspace_size(int64_t n, uint32_t max_knn)
+{
+    return search_ws_bytes((uint64_t)std::max<int64_t>(n, 0), (uint64_t)std::max<int64_t>(n, 0), max_knn, false,
+                           true);
+}
+
+int pgeof_knn_search_device(const float* data, int64_t n_data, const float* query, int64_t n_query, uint32_t knn,
+                            uint32_t* indices, float* sqr_dist, void* workspace, size_t workspace_bytes,
+                            void* stream)
+{
+    int rc = validate_knn(n_data, n_query, knn, false);
+    if (rc) return rc;
+    if (n_query == 0 || knn == 0) return PGEOF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    WsGuard g;
+    Arena ar;
+    rc = acquire_ws(g, workspace, workspace_bytes, pgeof_knn_workspace_size(n_data, n_query, knn), s, ar);
+    if (rc) return rc;
+    return run_search(kModeKnn, data, (uint32_t)n_data, query, (uint32_t)n_query, knn, 0.0f, indices, sqr_dist,
+                      nullptr, nullptr, ar, s);
+}
+
+int pgeof_radius_search_device(const float* data, int64_t n_data, const float* query, int64_t n_query,
+                               float search_radius, uint32_t max_knn, int32_t* indices, float* sqr_dist,
+                               void* workspace, size_t workspace_bytes, void* stream)
+{
+    int rc = validate_knn(n_data, n_query, max_knn, true);
+    if (rc) return rc;
+    if (n_query == 0 || max_knn == 0) return PGEOF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    WsGuard g;
+    Arena ar;
+    rc = acquire_ws(g, workspace, workspace_bytes, pgeof_radius_workspace_size(n_data, n_query, max_knn), s, ar);
+    if (rc) return rc;
+    return run_search(kModeRadius, data, (uint32_t)n_data, query, (uint32_t)n_query, max_knn, search_radius, indices,
+                      sqr_dist, nullptr, nullptr, ar, s);
+}
+
+int pgeof_knn_features_device(const float* xyz, int64_t n, uint32_t knn, uint32_t* nn, uint32_t* nn_ptr,
+                              float* sqr_dist, float* features, void* workspace, size_t workspace_bytes,
+                              void* stream)
+{
+    int rc = validate_knn(n, n, knn, false);
+    if (rc) return rc;
+    if (knn < 1) return fail(PGEOF_EINVAL, "knn should be >= 1");
+    if ((uint64_t)n * knn > 0xFFFFFFFFull) return fail(PGEOF_EINVAL, "n * knn exceeds the uint32 CSR range");
+    if (n == 0) return PGEOF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    WsGuard g;
+    Arena ar;
+    rc = acquire_ws(g, workspace, workspace_bytes, pgeof_knn_workspace_size(n, n, knn), s, ar);
+    if (rc) return rc;
+    rc = run_search(kModeKnnFeat, xyz, (uint32_t)n, xyz, (uint32_t)n, knn, 0.0f, nn, sqr_dist, features, nullptr, ar,
+                    s);
+    if (rc) return rc;
+    HIP_TRY(launch_arange_ptr(nn_ptr, (uint32_t)n, knn, s));
+    return PGEOF_OK;
+}
+
+int pgeof_radius_csr_device(const float* xyz, int64_t n, float search_radius, uint32_t max_knn, uint32_t* nn,
+                            uint32_t* nn_ptr, int64_t* nnz_out, void* workspace, size_t workspace_bytes,
+                            void* stream)
+{
+    int rc = validate_knn(n, n, max_knn, true);
+    if (rc) return rc;
+    if ((uint64_t)n * max_knn > 0xFFFFFFFFull) return fail(PGEOF_EINVAL, "n * max_knn exceeds the uint32 CSR range");
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0 || max_knn == 0)
+    {
+        HIP_TRY(launch_fill_u32(nn_ptr, (uint64_t)n + 1, 0u, s));
+        if (nnz_out) *nnz_out = 0;
+        return PGEOF_OK;
+    }
+    WsGuard g;
+    Arena ar;
+    rc = acquire_ws(g, workspace, workspace_bytes, pgeof_radius_csr_workspace_size(n, max_knn), s, ar);
+    if (rc) return rc;
+    SearchWs w;
+    rc = run_search(kModeRadiusCsr, xyz, (uint32_t)n, xyz, (uint32_t)n, max_knn, search_radius, nullptr, nullptr,
+                    nullptr, nullptr, ar, s, &w);
+    if (rc) return rc;
+    HIP_TRY(launch_fill_u32(w.counts + n, 1, 0u, s));
+    HIP_TRY(launch_scan_exclusive(w.counts, nn_ptr, (uint64_t)n + 1, w.csr_part, s));
+    HIP_TRY(launch_compact_rows(w.dense, (uint32_t)n, max_knn, w.counts, nn_ptr, nn, s));
+    if (nnz_out)
+    {
+        uint32_t nnz = 0;
+        rc = readback(&nnz, nn_ptr + n, 4, s);
+        if (rc) return rc;
+        *nnz_out = nnz;
+    }
+    return PGEOF_OK;
+}
+
+// ---------------------------------------------------- device features ----
+static int csr_device_checked(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                              const uint32_t* nn_ptr, int64_t n_nn_ptr, uint32_t k_min, uint32_t k_step,
+                              uint32_t k_min_search, const uint32_t* scales_host, int64_t n_scales, float* out,
+                              uint32_t* err, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (n_nn_ptr - 1 == 0) return PGEOF_OK;
+    uint32_t* d_sc = nullptr;
+    uint32_t* d_err = err;
+    void* tmp = nullptr;
+    const size_t need = al((size_t)n_scales * 4 + 4) + al(4);
+    HIP_TRY(hipMallocAsync(&tmp, need, s));
+    d_sc = (uint32_t*)tmp;
+    if (n_scales) HIP_TRY(hipMemcpyAsync(d_sc, scales_host, (size_t)n_scales * 4, hipMemcpyHostToDevice, s));
+    if (!d_err)
+    {
+        d_err = (uint32_t*)((char*)tmp + al((size_t)n_scales * 4 + 4));
+        HIP_TRY(launch_fill_u32(d_err, 1, 0u, s));
+    }
+    int rc = csr_device(mode, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, k_min, k_step, k_min_search, d_sc,
+                        (uint32_t)n_scales, out, d_err, s);
+    if (rc)
+    {
+        (void)hipFreeAsync(tmp, s);
+        return rc;
+    }
+    if (!err)
+    {
+        uint32_t w = 0;
+        rc = readback(&w, d_err, 4, s);
+        (void)hipFreeAsync(tmp, s);
+        if (rc) return rc;
+        return err_from_word(w);
+    }
+    HIP_TRY(hipFreeAsync(tmp, s));
+    return PGEOF_OK;
+}
+
+int pgeof_compute_features_device(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                  const uint32_t* nn_ptr, int64_t n_nn_ptr, uint64_t k_min, float* features,
+                                  uint32_t* err, void* stream)
+{
+    if (k_min < 1) return fail(PGEOF_EINVAL, "k_min should be > 1");
+    int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
+    if (rc) return rc;
+    const uint32_t km = (uint32_t)std::min<uint64_t>(k_min, 0xFFFFFFFFull);
+    return csr_device_checked(0, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, km, 1, 1, nullptr, 0, features, err,
+                              stream);
+}
+
+int pgeof_compute_features_multiscale_device(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                             const uint32_t* nn_ptr, int64_t n_nn_ptr,
+                                             const uint32_t* k_scales_host, int64_t n_scales, float* features,
+                                             uint32_t* err, void* stream)
+{
+    if (n_scales < 0 || !check_scales(k_scales_host, n_scales))
+        return fail(PGEOF_EINVAL, "k_scales should be > 1 and sorted in ascending order");
+    int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
+    if (rc) return rc;
+    if (n_scales == 0) return PGEOF_OK;
+    return csr_device_checked(1, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, 1, 1, 1, k_scales_host, n_scales, features,
+                              err, stream);
+}
+
+int pgeof_compute_features_optimal_device(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                          const uint32_t* nn_ptr, int64_t n_nn_ptr, uint32_t k_min,
+                                          uint32_t k_step, uint32_t k_min_search, float* features, uint32_t* err,
+                                          void* stream)
+{
+    if (k_min < 1 && k_min_search < 1) return fail(PGEOF_EINVAL, "k_min and k_min_search should be > 1");
+    if (k_step < 1) return fail(PGEOF_EINVAL, "k_step should be >= 1");
+    int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
+    if (rc) return rc;
+    return csr_device_checked(2, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, k_min, k_step, k_min_search, nullptr, 0,
+                              features, err, stream);
+}
+
+}  // extern "C"

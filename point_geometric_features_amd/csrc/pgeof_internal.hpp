@@ -1,0 +1,124 @@
+// Host-side interfaces between the C-ABI layer and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace pgeof_amd
+{
+
+// error bits reported by kernels through a device word
+enum : uint32_t
+{
+    kErrNnIndex = 1u,  // a neighbour index >= number of xyz rows
+    kErrNnPtr = 2u,    // nn_ptr not non-decreasing or beyond len(nn)
+};
+
+// Uniform grid over a point set.  Cells are linearised row-major with x
+// fastest, so one (y, z) row of cells is one contiguous range of the sorted
+// point array.
+struct GridParams
+{
+    double lo[3];
+    double h, inv_h;
+    int g[3];
+    uint64_t G;  // g[0] * g[1] * g[2]
+};
+
+struct GridView
+{
+    GridParams p;
+    const uint32_t* cell_start;  // G + 1 entries
+    const float4* pts;           // sorted points: x, y, z, bit pattern of the original index
+};
+
+// Arguments of the neighbour-search kernels.
+struct SearchArgs
+{
+    GridView grid;
+    const float4* queries;  // sorted queries (x, y, z, bits of original query index)
+    uint32_t n_query;
+    uint32_t k;    // kNN: k; radius: max_knn
+    float r2;      // radius: search_radius * search_radius (float)
+    float margin;  // absolute distance slack for cell pruning
+    void* out_idx;        // (n_query, k) u32 (kNN) or i32 (radius), rows = original query index
+    float* out_d2;        // (n_query, k) or nullptr
+    float* out_feat;      // (n_query, 11) or nullptr (fused features)
+    uint32_t* out_count;  // per original query: number of hits (radius) or nullptr
+    uint64_t* gheap_keys;  // global-memory heap fallback: k * n_query keys
+    uint32_t* gheap_pos;   // k * n_query positions
+};
+
+// Simple linear device arena carved out of one allocation.
+struct Arena
+{
+    char* base = nullptr;
+    size_t size = 0;
+    size_t off = 0;
+    template <typename T>
+    T* take(size_t count)
+    {
+        size_t a = (off + 255) & ~size_t(255);
+        off = a + count * sizeof(T);
+        return reinterpret_cast<T*>(base + a);
+    }
+};
+
+// ---- launchers (pgeof_search.hip) ----
+hipError_t launch_bbox(const float* xyz, uint32_t n, float* partial, uint32_t n_partial, float* out6,
+                       hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_hist(const float* xyz, uint32_t n, const GridParams& gp, uint32_t* count, uint32_t* cell,
+                       uint32_t* rank, hipStream_t s);
+hipError_t launch_sumsq(const uint32_t* count, uint64_t G, unsigned long long* out, hipStream_t s);
+size_t scan_partials_needed(uint64_t n);
+hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partials,
+                                 hipStream_t s);
+hipError_t launch_scatter(const float* xyz, uint32_t n, const uint32_t* cell, const uint32_t* rank,
+                          const uint32_t* cell_start, float4* pts, hipStream_t s);
+hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds_bytes, bool global_heap,
+                         hipStream_t s);
+hipError_t launch_arange_ptr(uint32_t* nn_ptr, uint32_t n_points, uint32_t k, hipStream_t s);
+hipError_t launch_compact_rows(const uint32_t* dense, uint32_t n_rows, uint32_t cap, const uint32_t* counts,
+                               const uint32_t* row_ptr, uint32_t* nn, hipStream_t s);
+
+enum SearchMode
+{
+    kModeKnn = 0,       // knn_search outputs
+    kModeRadius = 1,    // radius_search outputs (dense, -1 padded)
+    kModeKnnFeat = 2,   // fused: CSR row of neighbours + 11 features
+    kModeRadiusCsr = 3  // radius hits as u32 dense rows + counts (for CSR compaction)
+};
+
+// ---- launchers (pgeof_features.hip) ----
+struct CsrArgs
+{
+    const float* xyz;
+    uint32_t n_xyz;
+    const uint32_t* nn;
+    uint64_t n_nn;
+    const uint32_t* nn_ptr;
+    uint32_t n_points;
+    uint32_t k_min, k_step, k_min_search;
+    const uint32_t* scales;
+    uint32_t n_scales;
+    float* out;
+    uint32_t* err;
+};
+hipError_t launch_csr_features(int mode, const CsrArgs& a, hipStream_t s);
+
+// ---- profiling hooks (pgeof_capi.cpp) ----
+void prof_begin(const char* name, hipStream_t s, void** token);
+void prof_end(void* token, hipStream_t s);
+
+struct ProfScope
+{
+    void* tok = nullptr;
+    hipStream_t s;
+    ProfScope(const char* name, hipStream_t st) : s(st) { prof_begin(name, st, &tok); }
+    ~ProfScope() { prof_end(tok, s); }
+};
+
+}  // namespace pgeof_amd

@@ -1,0 +1,92 @@
+"""CPU-only checks of the boundary: the C-ABI library loads and exports every
+symbol include/pgeof_hip.h declares; the Python module exposes the reference's
+names, keywords and defaults (src/pgeof_ext.cpp:18-177); argument errors are
+raised on the host before any device work."""
+import inspect
+
+import numpy as np
+import pytest
+
+import point_geometric_features_amd as pgeof
+from point_geometric_features_amd import capi
+
+
+def test_library_exports_every_header_symbol():
+    L = capi.lib()
+    names = capi.header_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(capi._SIGS), set(names) ^ set(capi._SIGS)
+    assert L.pgeof_abi_version() == 100
+
+
+def test_module_surface_matches_reference():
+    for name in ("EFeatureID", "compute_features", "compute_features_multiscale", "compute_features_optimal",
+                 "knn_search", "radius_search", "compute_features_selected"):
+        assert hasattr(pgeof, name)
+    E = pgeof.EFeatureID
+    expect = ["Linearity", "Planarity", "Scattering", "VerticalityPGEOF", "Normal_x", "Normal_y", "Normal_z",
+              "Length", "Surface", "Volume", "Curvature", "K_optimal", "Verticality", "Eigentropy"]
+    for i, n in enumerate(expect):
+        assert int(getattr(E, n)) == i
+        assert int(getattr(pgeof, n)) == i  # export_values()
+
+
+def _doc_sig(fn):
+    return fn.__doc__.splitlines()[0]
+
+
+def test_keywords_and_defaults():
+    assert "k_min: object = 1" in _doc_sig(pgeof.compute_features) or "k_min" in _doc_sig(pgeof.compute_features)
+    xyz = np.zeros((4, 3), np.float32)
+    nn = np.zeros(4, np.uint32)
+    ptr = np.zeros(5, np.uint32)
+    # keyword names are accepted (argument errors are raised before any device work)
+    with pytest.raises(ValueError):
+        pgeof.compute_features(xyz=xyz, nn=nn, nn_ptr=ptr, k_min=0, verbose=False)
+    with pytest.raises(ValueError):
+        pgeof.compute_features_multiscale(xyz=xyz, nn=nn, nn_ptr=ptr, k_scales=[3, 2], verbose=False)
+    with pytest.raises(ValueError):
+        pgeof.compute_features_optimal(xyz=xyz, nn=nn, nn_ptr=ptr, k_min=0, k_step=1, k_min_search=0)
+    with pytest.raises(ValueError):
+        pgeof.knn_search(data=xyz, query=xyz, knn=5)
+    with pytest.raises(ValueError):
+        pgeof.radius_search(data=xyz, query=xyz, search_radius=0.1, max_knn=5)
+
+
+@pytest.mark.parametrize("bad", [np.zeros((4, 3)), np.zeros((4, 4), np.float32), np.zeros(12, np.float32),
+                                 [[0.0, 0.0, 0.0]]])
+def test_dtype_and_shape_errors_are_type_errors(bad):
+    nn = np.zeros(4, np.uint32)
+    ptr = np.zeros(5, np.uint32)
+    with pytest.raises(TypeError):
+        pgeof.compute_features(bad, nn, ptr)
+    with pytest.raises(TypeError):
+        pgeof.knn_search(bad, bad, 1)
+
+
+def test_index_dtype_errors():
+    xyz = np.zeros((4, 3), np.float32)
+    with pytest.raises(TypeError):
+        pgeof.compute_features(xyz, np.zeros(4, np.int32), np.zeros(5, np.uint32))
+    with pytest.raises(TypeError):
+        pgeof.compute_features(xyz, np.zeros(4, np.uint32), np.zeros(5, np.int64))
+    with pytest.raises(TypeError):
+        pgeof.compute_features(xyz, np.zeros((2, 2), np.uint32), np.zeros(5, np.uint32))
+
+
+def test_workspace_sizes_are_monotone():
+    a = capi.knn_workspace_size(10_000, 10_000, 20)
+    b = capi.knn_workspace_size(1_000_000, 1_000_000, 20)
+    assert 0 < a < b
+    assert capi.radius_csr_workspace_size(1000, 16) > 0
+
+
+def test_tuning_roundtrip():
+    old = capi.get_tuning("knn_occupancy")
+    capi.set_tuning("knn_occupancy", 0.2)
+    assert capi.get_tuning("knn_occupancy") == 0.2
+    capi.set_tuning("knn_occupancy", old)
+    with pytest.raises(ValueError):
+        capi.set_tuning("nope", 1.0)

@@ -1,0 +1,141 @@
+"""GPU parity of knn_search / radius_search against the CPU oracle.
+
+Bar: bit-exact indices AND squared distances vs the float32 brute-force /
+KD-tree oracle (oracle/pgeof_oracle.c, nanoflann's L2_Simple expression, ties
+by index); the reference's own tests (tests/test_pgeof.py:8-27 upstream)
+compare indices with scipy's KD-tree, reproduced here with fixed seeds.
+"""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+import point_geometric_features_amd as pgeof
+from oracle import oracle as O
+from point_geometric_features_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _eq_knn(data, query, k, kdtree=True):
+    gi, gd = pgeof.knn_search(data, query, k)
+    oi, od = O.knn(data, query, k, kdtree=kdtree)
+    assert gi.dtype == np.uint32 and gd.dtype == np.float32
+    assert gi.shape == (len(query), k)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    return gi, gd
+
+
+def test_knn_reference_test_vs_scipy():
+    # tests/test_pgeof.py:8-15 (reference), seeded
+    rng = np.random.default_rng(1234)
+    xyz = rng.uniform(0.0, 200.0, size=(1000, 3)).astype(np.float32)
+    _, k_legacy = cKDTree(xyz).query(xyz, k=10)
+    k_new, _ = pgeof.knn_search(xyz, xyz, 10)
+    np.testing.assert_equal(k_legacy, k_new)
+
+
+def test_knn_uniform_10k_k20_bitexact():
+    xyz = scenes.uniform_cube(10_000, seed=0)
+    gi, gd = _eq_knn(xyz, xyz, 20, kdtree=False)
+    assert (gi[:, 0] == np.arange(len(xyz))).all()
+    assert (np.diff(gd, axis=1) >= 0).all()
+
+
+@pytest.mark.parametrize("k", [1, 2, 7, 33, 64, 100])
+def test_knn_various_k(k):
+    xyz = scenes.uniform_cube(3000, seed=k)
+    _eq_knn(xyz, xyz, k, kdtree=False)
+
+
+def test_knn_large_k_global_heap():
+    # k too large for the LDS heap budget -> global-memory heap path
+    xyz = scenes.uniform_cube(1500, seed=5)
+    _eq_knn(xyz, xyz, 1200, kdtree=False)
+
+
+def test_knn_k_equals_n():
+    xyz = scenes.uniform_cube(300, seed=6)
+    _eq_knn(xyz, xyz, 300, kdtree=False)
+
+
+def test_knn_queries_differ_and_outside():
+    rng = np.random.default_rng(7)
+    data = rng.uniform(0, 10, size=(5000, 3)).astype(np.float32)
+    q = rng.uniform(-20, 30, size=(2000, 3)).astype(np.float32)
+    _eq_knn(data, q, 16, kdtree=False)
+
+
+def test_knn_duplicates_ties_by_index():
+    rng = np.random.default_rng(8)
+    base = rng.integers(0, 6, size=(400, 3)).astype(np.float32)  # lattice: many exact ties
+    xyz = np.concatenate([base, base[::-1]])
+    _eq_knn(xyz, xyz, 24, kdtree=False)
+
+
+def test_knn_lidar_200k_vs_kdtree():
+    xyz = scenes.lidar_scene(200_000, seed=11, extent=40.0)
+    _eq_knn(xyz, xyz, 20, kdtree=True)
+
+
+def test_knn_precision_scene_offset():
+    xyz = scenes.precision_scene(60_000, seed=12)
+    _eq_knn(xyz, xyz, 50, kdtree=True)
+
+
+def test_knn_empty_query_and_errors():
+    xyz = scenes.uniform_cube(100, seed=9)
+    gi, gd = pgeof.knn_search(xyz, np.zeros((0, 3), np.float32), 5)
+    assert gi.shape == (0, 5)
+    with pytest.raises(ValueError):
+        pgeof.knn_search(xyz, xyz, 101)
+    with pytest.raises(TypeError):
+        pgeof.knn_search(xyz.astype(np.float64), xyz, 5)
+
+
+def test_radius_reference_test_vs_scipy():
+    # tests/test_pgeof.py:18-27 (reference), seeded
+    rng = np.random.default_rng(4321)
+    xyz = rng.random(size=(1000, 3), dtype=np.float32)
+    _, k_legacy = cKDTree(xyz).query(xyz, k=10, distance_upper_bound=0.2)
+    k_legacy[k_legacy == xyz.shape[0]] = -1
+    k_new, _ = pgeof.radius_search(xyz, xyz, 0.2, 10)
+    np.testing.assert_equal(k_legacy, k_new)
+
+
+@pytest.mark.parametrize("r,cap", [(0.05, 20), (0.1, 64), (0.02, 5), (0.3, 150)])
+def test_radius_bitexact(r, cap):
+    xyz = scenes.uniform_cube(8000, seed=int(r * 1000) + cap)
+    gi, gd = pgeof.radius_search(xyz, xyz, r, cap)
+    oi, od, cnt = O.radius(xyz, xyz, r, cap, kdtree=False)
+    assert gi.dtype == np.int32
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    # nanoflann RKNNResultSet: r^2 in the last slot of rows not filled
+    notfull = cnt < cap
+    if notfull.any():
+        assert (gd[notfull, -1] == np.float32(r) * np.float32(r)).all()
+
+
+def test_radius_queries_differ():
+    rng = np.random.default_rng(13)
+    data = rng.uniform(0, 5, size=(6000, 3)).astype(np.float32)
+    q = rng.uniform(-1, 6, size=(3000, 3)).astype(np.float32)
+    gi, gd = pgeof.radius_search(data, q, 0.4, 32)
+    oi, od, _ = O.radius(data, q, 0.4, 32, kdtree=False)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+
+
+def test_radius_lidar_vs_kdtree():
+    xyz = scenes.lidar_scene(150_000, seed=14, extent=35.0)
+    gi, gd = pgeof.radius_search(xyz, xyz, 0.5, 64)
+    oi, od, _ = O.radius(xyz, xyz, 0.5, 64, kdtree=True)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+
+
+def test_radius_errors():
+    xyz = scenes.uniform_cube(50, seed=15)
+    with pytest.raises(ValueError):
+        pgeof.radius_search(xyz, xyz, 0.1, 51)
