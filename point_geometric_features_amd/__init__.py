@@ -12,6 +12,10 @@ import os as _os
 
 _HERE = _os.path.dirname(_os.path.abspath(__file__))
 
+from ._runtime import prefer_torch_hip_runtime as _prefer_runtime
+
+_prefer_runtime()
+
 try:
     from .pgeof_ext import (  # noqa: F401
         EFeatureID,

@@ -72,6 +72,9 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run `python -m point_geometric_features_amd._build`")
+        from ._runtime import prefer_torch_hip_runtime
+
+        prefer_torch_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
