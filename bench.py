@@ -36,6 +36,7 @@ def _parse():
     ap.add_argument("--workload", default="lidar10m_k20")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=400_000)
+    ap.add_argument("--tune", action="append", default=[], help="library tuning knob key=value")
     return ap.parse_args()
 
 
@@ -85,10 +86,15 @@ def main():
 
     from point_geometric_features_amd import capi, scenes
 
+    for kv in args.tune:
+        key, val = kv.split("=")
+        capi.set_tuning(key, float(val))
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     wl = args.workload
     cfg = {"workload": wl}
+    if args.tune:
+        cfg["tuning"] = dict(kv.split("=") for kv in args.tune)
     extra = {}
 
     if wl in ("lidar10m_k20", "lidar10m_k50"):

@@ -55,6 +55,7 @@ struct Tuning
     double grid_cells_per_point = 4.0;  // cap on G / n
     double lds_budget = 65536.0;        // bytes of LDS per workgroup for heaps
     double radius_cell_factor = 0.5;    // cell edge = this * radius
+    double grid_levels = 4.0;           // kNN grid pyramid depth (1..4)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -232,11 +233,11 @@ struct SearchWs
     float* bbox_out;
     unsigned long long* sumsq;
     uint32_t* count;
-    uint32_t* cell_start;
+    uint32_t* cs_all;  // cell starts of every level, 2 * (cap + 1) + 64 entries
     uint32_t* scan_part;
     uint32_t* cell;
     uint32_t* rank;
-    float4* pts;
+    float4* pts[kMaxLevels];
     uint32_t* qcell;
     uint32_t* qrank;
     float4* qs;
@@ -245,6 +246,10 @@ struct SearchWs
     uint32_t* dense;   // radius csr: n_query * cap
     uint32_t* counts;  // radius csr: n_query (+1 for the scan)
     uint32_t* csr_part;
+    uint32_t* flags;   // kNN: deferred queries of the budgeted pass (n_query + 1)
+    uint32_t* foff;    // their exclusive scan
+    uint32_t* fpart;
+    float4* qlist;     // compacted deferred queries
 };
 
 constexpr uint32_t kBboxBlocks = 1024;
@@ -256,13 +261,26 @@ struct HeapPlan
     bool global;
 };
 
+HeapPlan heap_plan_bytes(uint32_t k, size_t entry_bytes)
+{
+    const Tuning t = tuning();
+    const size_t per = (size_t)k * entry_bytes;
+    const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
+    for (int b : {256, 128, 64})
+        if (per * (size_t)b <= budget) return {b, per * (size_t)b, false};
+    return {256, 0, true};
+}
+
+// Search kernels: per wave 64 heaps of k entries plus a 128-point staging
+// buffer (2 KiB) in LDS.
 HeapPlan heap_plan(uint32_t k, bool with_pos)
 {
     const Tuning t = tuning();
     const size_t per = (size_t)k * (8 + (with_pos ? 4 : 0));
+    const size_t stage = 128 * 16;
     const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
     for (int b : {256, 128, 64})
-        if (per * (size_t)b <= budget) return {b, per * (size_t)b, false};
+        if (per * (size_t)b + stage * (size_t)(b / 64) <= budget) return {b, per * (size_t)b + stage * (size_t)(b / 64), false};
     return {256, 0, true};
 }
 
@@ -271,9 +289,10 @@ size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool with_pos, bool c
     const uint64_t G = grid_cap(n) + 1;
     size_t b = 0;
     b += al(kBboxBlocks * 6 * sizeof(float)) + al(64) + al(64);
-    b += 2 * al(G * 4) + al(scan_partials_needed(G) * 4);
-    b += 2 * al(n * 4) + al(n * 16);
+    b += al(G * 4) + al((2 * G + 64) * 4) + al(scan_partials_needed(G) * 4);
+    b += 2 * al(n * 4) + kMaxLevels * al(n * 16);
     b += 2 * al(m * 4) + al(m * 16);
+    b += 2 * al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4) + al(m * 16);
     const HeapPlan hp = heap_plan(k, with_pos);
     if (hp.global) b += al(m * k * 8) + al(m * k * 4);
     if (csr) b += al(m * k * 4) + al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4);
@@ -288,14 +307,18 @@ SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool with_pos, boo
     w.bbox_out = ar.take<float>(16);
     w.sumsq = ar.take<unsigned long long>(8);
     w.count = ar.take<uint32_t>(G);
-    w.cell_start = ar.take<uint32_t>(G);
+    w.cs_all = ar.take<uint32_t>(2 * G + 64);
     w.scan_part = ar.take<uint32_t>(scan_partials_needed(G));
     w.cell = ar.take<uint32_t>(n);
     w.rank = ar.take<uint32_t>(n);
-    w.pts = ar.take<float4>(n);
+    for (int l = 0; l < kMaxLevels; ++l) w.pts[l] = ar.take<float4>(n);
     w.qcell = ar.take<uint32_t>(m);
     w.qrank = ar.take<uint32_t>(m);
     w.qs = ar.take<float4>(m);
+    w.flags = ar.take<uint32_t>(m + 1);
+    w.foff = ar.take<uint32_t>(m + 1);
+    w.fpart = ar.take<uint32_t>(scan_partials_needed(m + 1));
+    w.qlist = ar.take<float4>(m);
     const HeapPlan hp = heap_plan(k, with_pos);
     if (hp.global)
     {
@@ -322,11 +345,12 @@ int readback(void* host, const void* dev, size_t bytes, hipStream_t s)
     return PGEOF_OK;
 }
 
-// Build the sorted grid of `data` and (when query != data) sort the queries
-// into the same grid.  mode: kModeKnn/kModeKnnFeat (k used for sizing) or a
-// radius mode (radius used for sizing).
+// Sort `data` into a pyramid of uniform grids (cell edge h_0 * 4^l) and,
+// when query != data, sort the queries into level 0.  kNN sizes h_0 from the
+// target occupancy (k used for sizing); radius searches use one level with
+// cells of radius_cell_factor * radius.
 int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, uint32_t k, float radius, bool is_radius,
-               SearchWs& w, GridView& gv, const float4** qs_out, hipStream_t s)
+               SearchWs& w, SearchArgs& sa, const float4** qs_out, hipStream_t s)
 {
     const uint32_t nbb = std::max<uint32_t>(1, std::min<uint32_t>(kBboxBlocks, (n + 255) / 256));
     HIP_TRY(launch_bbox(data, n, w.bbox_partial, nbb, w.bbox_out, s));
@@ -376,36 +400,57 @@ int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, ui
         if (m1 > T) h = h1 * std::sqrt(T / m1);
         h = std::max(h, hcap);
     }
-    gv.p = make_grid(bb, h);
-    const uint64_t G = gv.p.G;
-    HIP_TRY(launch_fill_u32(w.count, G + 1, 0u, s));
-    HIP_TRY(launch_hist(data, n, gv.p, w.count, w.cell, w.rank, s));
-    HIP_TRY(launch_scan_exclusive(w.count, w.cell_start, G + 1, w.scan_part, s));
-    HIP_TRY(launch_scatter(data, n, w.cell, w.rank, w.cell_start, w.pts, s));
-    gv.cell_start = w.cell_start;
-    gv.pts = w.pts;
+    // levels
+    const int max_levels = is_radius ? 1 : std::max(1, std::min(kMaxLevels, (int)t.grid_levels));
+    uint64_t cs_used = 0;
+    const uint64_t cs_room = 2 * (cap + 1) + 64;
+    sa.n_levels = 0;
+    for (int l = 0; l < max_levels; ++l)
+    {
+        const double hl = h * std::pow(4.0, (double)l);
+        const GridParams gp = make_grid(bb, hl);
+        if (l > 0 && (cs_used + gp.G + 1 > cs_room || sa.lv[l - 1].g[0] * (uint64_t)sa.lv[l - 1].g[1] *
+                                                              (uint64_t)sa.lv[l - 1].g[2] <= 64))
+            break;
+        uint32_t* csl = w.cs_all + cs_used;
+        cs_used += gp.G + 1;
+        HIP_TRY(launch_fill_u32(w.count, gp.G + 1, 0u, s));
+        HIP_TRY(launch_hist(data, n, gp, w.count, w.cell, w.rank, s));
+        HIP_TRY(launch_scan_exclusive(w.count, csl, gp.G + 1, w.scan_part, s));
+        HIP_TRY(launch_scatter(data, n, w.cell, w.rank, csl, w.pts[l], s));
+        Level& L = sa.lv[l];
+        for (int a = 0; a < 3; ++a)
+        {
+            L.lo[a] = gp.lo[a];
+            L.g[a] = gp.g[a];
+        }
+        L.h = gp.h;
+        L.inv_h = gp.inv_h;
+        L.cs = csl;
+        L.pts = w.pts[l];
+        sa.n_levels = l + 1;
+    }
+    const Level& L0 = sa.lv[0];
     if (query == data && m == n)
-        *qs_out = w.pts;
+        *qs_out = w.pts[0];
     else
     {
-        HIP_TRY(launch_fill_u32(w.count, G + 1, 0u, s));
-        HIP_TRY(launch_hist(query, m, gv.p, w.count, w.qcell, w.qrank, s));
-        HIP_TRY(launch_scan_exclusive(w.count, w.count, G + 1, w.scan_part, s));
+        GridParams g0;
+        for (int a = 0; a < 3; ++a)
+        {
+            g0.lo[a] = L0.lo[a];
+            g0.g[a] = L0.g[a];
+        }
+        g0.h = L0.h;
+        g0.inv_h = L0.inv_h;
+        g0.G = (uint64_t)L0.g[0] * L0.g[1] * L0.g[2];
+        HIP_TRY(launch_fill_u32(w.count, g0.G + 1, 0u, s));
+        HIP_TRY(launch_hist(query, m, g0, w.count, w.qcell, w.qrank, s));
+        HIP_TRY(launch_scan_exclusive(w.count, w.count, g0.G + 1, w.scan_part, s));
         HIP_TRY(launch_scatter(query, m, w.qcell, w.qrank, w.count, w.qs, s));
         *qs_out = w.qs;
     }
     return PGEOF_OK;
-}
-
-SearchArgs base_args(const GridView& gv, const float4* qs, uint32_t m, uint32_t k)
-{
-    SearchArgs a{};
-    a.grid = gv;
-    a.queries = qs;
-    a.n_query = m;
-    a.k = k;
-    a.margin = (float)(gv.p.h * 1e-6);
-    return a;
 }
 
 // Device search core shared by the host and device entry points.
@@ -417,12 +462,15 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     const bool csr = mode == kModeRadiusCsr;
     SearchWs w = carve(ar, n, m, k, with_pos, csr);
     if (ar.off > ar.size) return fail(PGEOF_ENOMEM, "workspace too small");
-    GridView gv;
+    SearchArgs a{};
     const float4* qs = nullptr;
     const bool is_radius = mode == kModeRadius || mode == kModeRadiusCsr;
-    int rc = build_grid(data, n, query, m, k, radius, is_radius, w, gv, &qs, s);
+    int rc = build_grid(data, n, query, m, k, radius, is_radius, w, a, &qs, s);
     if (rc) return rc;
-    SearchArgs a = base_args(gv, qs, m, k);
+    a.queries = qs;
+    a.n_query = m;
+    a.k = k;
+    a.margin = 1e-6f;
     a.r2 = radius * radius;
     a.out_idx = csr ? (void*)w.dense : out_idx;
     a.out_d2 = out_d2;
@@ -431,7 +479,30 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     const HeapPlan hp = heap_plan(k, with_pos);
     a.gheap_keys = w.gheap_keys;
     a.gheap_pos = w.gheap_pos;
-    HIP_TRY(launch_search(mode, a, hp.block, hp.lds, hp.global, s));
+    if (mode == kModeKnn || mode == kModeKnnFeat)
+    {
+        // pass A: level 0 with a bounded cell budget; queries whose k-th
+        // neighbour lies farther out are flagged and searched again, compacted
+        // and in sorted order, on the full grid pyramid (pass B)
+        a.defer_flag = w.flags;
+        HIP_TRY(launch_search(mode, a, hp.block, hp.lds, hp.global, true, s));
+        HIP_TRY(launch_fill_u32(w.flags + m, 1, 0u, s));
+        HIP_TRY(launch_scan_exclusive(w.flags, w.foff, (uint64_t)m + 1, w.fpart, s));
+        uint32_t n_def = 0;
+        rc = readback(&n_def, w.foff + m, 4, s);
+        if (rc) return rc;
+        if (n_def)
+        {
+            HIP_TRY(launch_compact_queries(qs, m, w.flags, w.foff, w.qlist, s));
+            SearchArgs b = a;
+            b.queries = w.qlist;
+            b.n_query = n_def;
+            b.defer_flag = nullptr;
+            HIP_TRY(launch_search(mode, b, hp.block, hp.lds, hp.global, false, s));
+        }
+    }
+    else
+        HIP_TRY(launch_search(mode, a, hp.block, hp.lds, hp.global, false, s));
     if (ws_out) *ws_out = w;
     return PGEOF_OK;
 }
@@ -561,6 +632,79 @@ int acquire_ws(WsGuard& g, void* ws, size_t ws_bytes, size_t need, hipStream_t s
     return PGEOF_OK;
 }
 
+// compute_features_selected on host arrays (include/pgeof.hpp:325-375).
+template <typename T>
+int selected_host(const T* xyz, int64_t n, T search_radius, uint32_t max_knn, const int32_t* selected,
+                  int64_t n_selected, T* out)
+{
+    if (!ok_size(n) || n_selected < 0) return fail(PGEOF_EINVAL, "input too large");
+    for (int64_t i = 0; i < n_selected; ++i)
+        if (selected[i] < 0 || selected[i] > 13) return fail(PGEOF_EINVAL, "selected_features holds a value outside EFeatureID");
+    if (n == 0 || n_selected == 0) return PGEOF_OK;
+    const bool f64 = sizeof(T) == 8;
+    const uint32_t cap = std::max<uint32_t>(max_knn, 2);
+    hipStream_t s;
+    int rc = ctx_prepare(&s);
+    if (rc) return rc;
+    const HeapPlan hp = heap_plan_bytes(cap, sizeof(T) + 4);
+    const size_t b_x = al((size_t)n * 3 * sizeof(T)), b_xf = f64 ? al((size_t)n * 12) : 0,
+                 b_o = al((size_t)n * (size_t)n_selected * sizeof(T)), b_f = al((size_t)n_selected * 4),
+                 b_h = hp.global ? al((size_t)n * cap * sizeof(T)) + al((size_t)n * cap * 4) : 0;
+    const size_t b_ws = search_ws_bytes((uint64_t)n, (uint64_t)n, 1, false, false);
+    rc = ctx_reserve(b_x + b_xf + b_o + b_f + b_h + b_ws);
+    if (rc) return rc;
+    Arena ar{t_ctx.dbuf, t_ctx.dbytes, 0};
+    T* d_x = ar.take<T>((size_t)n * 3);
+    float* d_xf = f64 ? ar.take<float>((size_t)n * 3) : reinterpret_cast<float*>(d_x);
+    T* d_out = ar.take<T>((size_t)n * (size_t)n_selected);
+    int32_t* d_f = ar.take<int32_t>((size_t)n_selected);
+    void* d_hk = hp.global ? (void*)ar.take<T>((size_t)n * cap) : nullptr;
+    uint32_t* d_hi = hp.global ? ar.take<uint32_t>((size_t)n * cap) : nullptr;
+    HIP_TRY(hipMemcpyAsync(d_x, xyz, (size_t)n * 3 * sizeof(T), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_f, selected, (size_t)n_selected * 4, hipMemcpyHostToDevice, s));
+    if (f64) HIP_TRY(launch_f64_to_f32(reinterpret_cast<const double*>(d_x), d_xf, (uint64_t)n * 3, s));
+    Arena wa{t_ctx.dbuf + ar.off, t_ctx.dbytes - ar.off, 0};
+    SearchWs w = carve(wa, (uint64_t)n, (uint64_t)n, 1, false, false);
+    if (wa.off > wa.size) return fail(PGEOF_ENOMEM, "workspace too small");
+    SearchArgs sa{};
+    const float4* qs = nullptr;
+    rc = build_grid(d_xf, (uint32_t)n, d_xf, (uint32_t)n, cap, (float)search_radius, true, w, sa, &qs, s);
+    if (rc) return rc;
+    GridView gv;
+    for (int ax = 0; ax < 3; ++ax)
+    {
+        gv.p.lo[ax] = sa.lv[0].lo[ax];
+        gv.p.g[ax] = sa.lv[0].g[ax];
+    }
+    gv.p.h = sa.lv[0].h;
+    gv.p.inv_h = sa.lv[0].inv_h;
+    gv.p.G = (uint64_t)sa.lv[0].g[0] * sa.lv[0].g[1] * sa.lv[0].g[2];
+    gv.cell_start = sa.lv[0].cs;
+    gv.pts = sa.lv[0].pts;
+    SelectedArgs a{};
+    a.grid = gv;
+    a.xyz = d_x;
+    a.n = (uint32_t)n;
+    a.cap = cap;
+    a.max_knn = max_knn;
+    a.r2 = (double)(search_radius * search_radius);
+    double maxabs = 0.0;
+    for (int ax = 0; ax < 3; ++ax)
+        maxabs = std::max(maxabs, std::max(std::fabs(gv.p.lo[ax]), std::fabs(gv.p.lo[ax] + gv.p.g[ax] * gv.p.h)));
+    // the grid holds float32 copies: widen the cell range by their rounding
+    const double margin = gv.p.h * 1e-6 + (f64 ? 2.4e-7 * maxabs : 0.0);
+    a.reach = std::fabs((double)search_radius) + margin;
+    a.fids = d_f;
+    a.n_sel = (uint32_t)n_selected;
+    a.out = d_out;
+    a.gheap_keys = d_hk;
+    a.gheap_idx = d_hi;
+    HIP_TRY(launch_selected(f64, a, hp.block, hp.lds, hp.global, s));
+    HIP_TRY(hipMemcpyAsync(out, d_out, (size_t)n * (size_t)n_selected * sizeof(T), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return PGEOF_OK;
+}
+
 }  // namespace
 
 // ===================================================== profiling hooks ====
@@ -618,6 +762,8 @@ int pgeof_set_tuning(const char* key, double v)
         g_tune.lds_budget = v;
     else if (k == "radius_cell_factor")
         g_tune.radius_cell_factor = v;
+    else if (k == "grid_levels")
+        g_tune.grid_levels = v;
     else
         return fail(PGEOF_EINVAL, "unknown tuning key " + k);
     return PGEOF_OK;
@@ -631,6 +777,7 @@ double pgeof_get_tuning(const char* key)
     if (k == "grid_cells_per_point") return g_tune.grid_cells_per_point;
     if (k == "lds_budget") return g_tune.lds_budget;
     if (k == "radius_cell_factor") return g_tune.radius_cell_factor;
+    if (k == "grid_levels") return g_tune.grid_levels;
     return -1.0;
 }
 
@@ -785,13 +932,13 @@ int pgeof_compute_features_optimal(const float* xyz, int64_t n_xyz, const uint32
 int pgeof_compute_features_selected_f32(const float* xyz, int64_t n, float search_radius, uint32_t max_knn,
                                         const int32_t* selected, int64_t n_selected, float* features)
 {
-    return fail(PGEOF_ERUNTIME, "compute_features_selected: not available in this build");
+    return selected_host<float>(xyz, n, search_radius, max_knn, selected, n_selected, features);
 }
 
 int pgeof_compute_features_selected_f64(const double* xyz, int64_t n, double search_radius, uint32_t max_knn,
                                         const int32_t* selected, int64_t n_selected, double* features)
 {
-    return fail(PGEOF_ERUNTIME, "compute_features_selected: not available in this build");
+    return selected_host<double>(xyz, n, search_radius, max_knn, selected, n_selected, features);
 }
 
 // ---------------------------------------------------- device searches ----

@@ -34,21 +34,35 @@ struct GridView
     const float4* pts;           // sorted points: x, y, z, bit pattern of the original index
 };
 
+// One level of the multi-level grid used by the neighbour search.
+struct Level
+{
+    double lo[3];
+    double h, inv_h;
+    int g[3];
+    const uint32_t* cs;  // cell starts, G + 1 entries
+    const float4* pts;   // points sorted by cell of this level
+};
+
+constexpr int kMaxLevels = 4;
+
 // Arguments of the neighbour-search kernels.
 struct SearchArgs
 {
-    GridView grid;
+    Level lv[kMaxLevels];  // level l has cell edge h_0 * 4^l
+    int n_levels;
     const float4* queries;  // sorted queries (x, y, z, bits of original query index)
     uint32_t n_query;
     uint32_t k;    // kNN: k; radius: max_knn
     float r2;      // radius: search_radius * search_radius (float)
-    float margin;  // absolute distance slack for cell pruning
+    float margin;  // absolute distance slack for cell pruning, relative to h (times h_l)
     void* out_idx;        // (n_query, k) u32 (kNN) or i32 (radius), rows = original query index
     float* out_d2;        // (n_query, k) or nullptr
     float* out_feat;      // (n_query, 11) or nullptr (fused features)
     uint32_t* out_count;  // per original query: number of hits (radius) or nullptr
     uint64_t* gheap_keys;  // global-memory heap fallback: k * n_query keys
     uint32_t* gheap_pos;   // k * n_query positions
+    uint32_t* defer_flag;  // budgeted pass: 1 where the query is left to the multi-level pass
 };
 
 // Simple linear device arena carved out of one allocation.
@@ -78,11 +92,31 @@ hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, 
                                  hipStream_t s);
 hipError_t launch_scatter(const float* xyz, uint32_t n, const uint32_t* cell, const uint32_t* rank,
                           const uint32_t* cell_start, float4* pts, hipStream_t s);
-hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds_bytes, bool global_heap,
+hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds_bytes, bool global_heap, bool budget,
                          hipStream_t s);
+hipError_t launch_compact_queries(const float4* q, uint32_t n, const uint32_t* flag, const uint32_t* off,
+                                  float4* out, hipStream_t s);
 hipError_t launch_arange_ptr(uint32_t* nn_ptr, uint32_t n_points, uint32_t k, hipStream_t s);
 hipError_t launch_compact_rows(const uint32_t* dense, uint32_t n_rows, uint32_t cap, const uint32_t* counts,
                                const uint32_t* row_ptr, uint32_t* nn, hipStream_t s);
+
+struct SelectedArgs
+{
+    GridView grid;       // grid over float32 copies of the points, self queries
+    const void* xyz;     // original coordinates (float or double), (n, 3)
+    uint32_t n;
+    uint32_t cap;        // heap capacity: max(max_knn, 2)
+    uint32_t max_knn;
+    double r2;           // search_radius^2 evaluated in the input precision
+    double reach;        // search_radius + margin, for the cell range
+    const int32_t* fids;  // EFeatureID list
+    uint32_t n_sel;
+    void* out;            // (n, n_sel) in the input precision
+    void* gheap_keys;     // global heap fallback: cap * n keys
+    uint32_t* gheap_idx;  // cap * n indices
+};
+hipError_t launch_selected(bool f64, const SelectedArgs& a, int block, size_t lds, bool global_heap, hipStream_t s);
+hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s);
 
 enum SearchMode
 {

@@ -272,21 +272,13 @@ struct Heap
     uint32_t* pos;
     uint32_t stride;
     uint32_t k;
+    uint32_t count;  // entries held; the heap property holds once count == k
 
-    __device__ __forceinline__ void init(uint64_t sentinel)
-    {
-        for (uint32_t j = 0; j < k; ++j)
-        {
-            keys[j * stride] = sentinel;
-            if (POS) pos[j * stride] = 0xFFFFFFFFu;
-        }
-    }
     __device__ __forceinline__ uint64_t top() const { return keys[0]; }
 
-    // put `key` at the root of heap [0, n) and restore the heap property
-    __device__ __forceinline__ void sift_root(uint64_t key, uint32_t p, uint32_t n)
+    // put `key` at index i of heap [0, n) and sift it down
+    __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t p, uint32_t n)
     {
-        uint32_t i = 0;
         for (;;)
         {
             const uint32_t l = 2 * i + 1;
@@ -311,8 +303,44 @@ struct Heap
         if (POS) pos[i * stride] = p;
     }
 
-    __device__ __forceinline__ void sort_ascending()
+    __device__ __forceinline__ void heapify()
     {
+        for (int i = (int)k / 2 - 1; i >= 0; --i)
+            sift_from((uint32_t)i, keys[i * stride], POS ? pos[i * stride] : 0u, k);
+    }
+
+    // precondition: key < worst (worst = sentinel until the heap is full)
+    __device__ __forceinline__ void add(uint64_t key, uint32_t p, uint64_t& worst)
+    {
+        if (count < k)
+        {
+            keys[count * stride] = key;
+            if (POS) pos[count * stride] = p;
+            if (++count == k)
+            {
+                heapify();
+                worst = keys[0];
+            }
+        }
+        else
+        {
+            sift_from(0, key, p, k);
+            worst = keys[0];
+        }
+    }
+
+    // pad with the sentinel, restore the heap property, sort ascending
+    __device__ __forceinline__ void finish(uint64_t sentinel)
+    {
+        if (count < k)
+        {
+            for (uint32_t j = count; j < k; ++j)
+            {
+                keys[j * stride] = sentinel;
+                if (POS) pos[j * stride] = 0xFFFFFFFFu;
+            }
+            heapify();
+        }
         for (uint32_t end = k - 1; end > 0; --end)
         {
             const uint64_t top_k = keys[0];
@@ -325,7 +353,7 @@ struct Heap
             }
             keys[end * stride] = top_k;
             if (POS) pos[end * stride] = tp;
-            sift_root(last, lp, end);
+            sift_from(0, last, lp, end);
         }
     }
 };
@@ -339,160 +367,365 @@ __device__ __forceinline__ double axis_gap(double v, int c, double lo, double h,
 }
 
 // ------------------------------------------------------------- search ----
-template <int MODE, int BLOCK, bool GLOBAL_HEAP>
-__global__ __launch_bounds__(BLOCK) void k_search(SearchArgs a)
+// One thread per query, queries in cell-sorted order.  Each query first
+// picks the finest grid level whose 3x3x3 cell cube around it already holds
+// k points (kNN; radius searches use level 0), so its row count stays small
+// whatever the local density.  Rows are then scanned in one of two ways,
+// chosen per wavefront and pass:
+//   - cooperative: when all lanes of the wave use the same level and the
+//     union of their rows is compact, the wave walks that union in lockstep,
+//     stages each row's point slice into LDS with coalesced loads and every
+//     lane scans its own sub-range from LDS;
+//   - per lane: otherwise each lane walks its own rows with batched loads.
+constexpr int kStage = 128;  // float4 points staged per wave per chunk
+constexpr int kBatch = 4;    // candidate loads in flight per lane (per-lane path)
+
+// Wavefront reductions over the active lanes (ROCm device library, DPP based).
+extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
+extern "C" __device__ uint32_t __ockl_wfred_max_u32(uint32_t);
+extern "C" __device__ int32_t __ockl_wfred_min_i32(int32_t);
+extern "C" __device__ int32_t __ockl_wfred_max_i32(int32_t);
+
+__device__ __forceinline__ int wave_min_i(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max_i(int v) { return __ockl_wfred_max_i32(v); }
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) { return __ockl_wfred_min_u32(v); }
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v) { return __ockl_wfred_max_u32(v); }
+
+// LDS visibility between the lanes of one wavefront
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct Box
+{
+    int x0, x1, y0, y1, z0, z1;  // inclusive cell ranges; empty when x0 > x1
+};
+
+__device__ __forceinline__ Box empty_box() { return Box{1, 0, 1, 0, 1, 0}; }
+
+__device__ __forceinline__ Box cube_box(int cx, int cy, int cz, int n, const Level& G)
+{
+    return Box{max(cx - n, 0), min(cx + n, G.g[0] - 1), max(cy - n, 0), min(cy + n, G.g[1] - 1), max(cz - n, 0),
+               min(cz + n, G.g[2] - 1)};
+}
+
+__device__ __forceinline__ Level pick_level(const SearchArgs& a, int l)
+{
+    Level G = a.lv[0];
+    if (l == 1) G = a.lv[1];
+    if (l == 2) G = a.lv[2];
+    if (l == 3) G = a.lv[3];
+    return G;
+}
+
+template <int MODE, int WAVES, bool GLOBAL_HEAP, bool BUDGET>
+__global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
 {
     constexpr bool POS = (MODE == kModeKnnFeat);
     constexpr bool RADIUS = (MODE == kModeRadius || MODE == kModeRadiusCsr);
+    constexpr int BLOCK = WAVES * 64;
     extern __shared__ __align__(16) unsigned char smem[];
 
-    const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const uint32_t t = lb * BLOCK + threadIdx.x;
-    if (t >= a.n_query) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t t = xcd_logical_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const bool active = t < a.n_query;
+    float4* stage = reinterpret_cast<float4*>(smem) + wid * kStage;
+    unsigned char* hbase = smem + (size_t)WAVES * kStage * sizeof(float4);
 
     Heap<POS> heap;
     if (GLOBAL_HEAP)
     {
-        heap.keys = a.gheap_keys + t;
-        heap.pos = a.gheap_pos + t;
+        // inactive lanes never touch their (non-existent) global heap
+        heap.keys = a.gheap_keys + (active ? t : 0);
+        heap.pos = a.gheap_pos + (active ? t : 0);
         heap.stride = a.n_query;
     }
     else
     {
-        heap.keys = reinterpret_cast<uint64_t*>(smem) + threadIdx.x;
-        heap.pos = reinterpret_cast<uint32_t*>(smem + (size_t)a.k * BLOCK * sizeof(uint64_t)) + threadIdx.x;
+        heap.keys = reinterpret_cast<uint64_t*>(hbase) + threadIdx.x;
+        heap.pos = reinterpret_cast<uint32_t*>(hbase + (size_t)a.k * BLOCK * sizeof(uint64_t)) + threadIdx.x;
         heap.stride = BLOCK;
     }
     heap.k = a.k;
+    heap.count = 0;
 
-    const float4 q4 = a.queries[t];
+    const float4 q4 = active ? a.queries[t] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t qorig = __float_as_uint(q4.w);
     const uint64_t sentinel = RADIUS ? ((uint64_t)__float_as_uint(a.r2) << 32) : ~0ull;
-    heap.init(sentinel);
     uint64_t worst = sentinel;
 
-    const GridParams& gp = a.grid.p;
-    const float4* __restrict__ pts = a.grid.pts;
-    const uint32_t* __restrict__ cs = a.grid.cell_start;
-    const bool finite = isfinite(q4.x) && isfinite(q4.y) && isfinite(q4.z);
+    bool live = active && isfinite(q4.x) && isfinite(q4.y) && isfinite(q4.z);
+    bool deferred = false;  // BUDGET: query handed over to the unbudgeted pass
+    const float qxf = q4.x, qyf = q4.y, qzf = q4.z;
+    const double qx = qxf, qy = qyf, qz = qzf;
 
-    if (finite)
+    // ---- level: finest one whose 3x3x3 cube holds k points ----
+    int level = 0;
+    if (!RADIUS && !BUDGET && live)
     {
-        const float qxf = q4.x, qyf = q4.y, qzf = q4.z;
-        auto scan = [&](uint32_t s, uint32_t e) {
-            for (uint32_t j = s; j < e; ++j)
-            {
-                const float4 p = pts[j];
-                const float d2 = sqdist32(qxf, qyf, qzf, p.x, p.y, p.z);
-                const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
-                if (key < worst)
-                {
-                    heap.sift_root(key, j, heap.k);
-                    worst = heap.top();
-                }
-            }
-        };
-        const uint32_t gx = (uint32_t)gp.g[0], gy = (uint32_t)gp.g[1];
-        auto row = [&](int y, int z, int x0, int x1) {
-            const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
-            scan(cs[base + (uint32_t)x0], cs[base + (uint32_t)x1 + 1]);
-        };
-        const double qx = qxf, qy = qyf, qz = qzf;
-        const int G0 = gp.g[0], G1 = gp.g[1], G2 = gp.g[2];
-        const int cx = cell_coord(qx, gp.lo[0], gp.inv_h, G0);
-        const int cy = cell_coord(qy, gp.lo[1], gp.inv_h, G1);
-        const int cz = cell_coord(qz, gp.lo[2], gp.inv_h, G2);
-
-        // Phase 1 (kNN): scan cubic shells around the query's cell until k
-        // candidates are held.
-        int n_done = -1;
-        bool covered = false;
-        if (!RADIUS)
+        for (; level < a.n_levels - 1; ++level)
         {
-            for (int n = 0;; ++n)
-            {
-                const int z0 = max(cz - n, 0), z1 = min(cz + n, G2 - 1);
-                const int y0 = max(cy - n, 0), y1 = min(cy + n, G1 - 1);
-                const int x0 = max(cx - n, 0), x1 = min(cx + n, G0 - 1);
-                for (int z = z0; z <= z1; ++z)
+            const Level L = pick_level(a, level);
+            const int lx = cell_coord(qx, L.lo[0], L.inv_h, L.g[0]);
+            const int ly = cell_coord(qy, L.lo[1], L.inv_h, L.g[1]);
+            const int lz = cell_coord(qz, L.lo[2], L.inv_h, L.g[2]);
+            const int x0 = max(lx - 1, 0), x1 = min(lx + 1, L.g[0] - 1);
+            uint32_t count = 0;
+            for (int z = max(lz - 1, 0); z <= min(lz + 1, L.g[2] - 1); ++z)
+                for (int y = max(ly - 1, 0); y <= min(ly + 1, L.g[1] - 1); ++y)
                 {
-                    const bool ze = (z == cz - n) || (z == cz + n);
-                    for (int y = y0; y <= y1; ++y)
+                    const uint32_t base = ((uint32_t)z * (uint32_t)L.g[1] + (uint32_t)y) * (uint32_t)L.g[0];
+                    count += L.cs[base + (uint32_t)x1 + 1] - L.cs[base + (uint32_t)x0];
+                }
+            if (count >= a.k) break;
+        }
+    }
+    const Level G = pick_level(a, level);
+    const float4* __restrict__ pts = G.pts;
+    const uint32_t* __restrict__ cs = G.cs;
+    const int G0 = G.g[0], G1 = G.g[1], G2 = G.g[2];
+    const uint32_t gx = (uint32_t)G0, gy = (uint32_t)G1;
+    const int cx = live ? cell_coord(qx, G.lo[0], G.inv_h, G0) : 0;
+    const int cy = live ? cell_coord(qy, G.lo[1], G.inv_h, G1) : 0;
+    const int cz = live ? cell_coord(qz, G.lo[2], G.inv_h, G2) : 0;
+    const double h = G.h, margin = G.h * (double)a.margin;
+
+    auto wlim = [&]() -> double {
+        if (worst == ~0ull) return INFINITY;
+        const float w = __uint_as_float((uint32_t)(worst >> 32));
+        if (!(w < INFINITY)) return INFINITY;
+        return (double)w * 1.00001 + 1e-37;
+    };
+
+    auto consider = [&](const float4& p, uint32_t j) {
+        const float d2 = sqdist32(qxf, qyf, qzf, p.x, p.y, p.z);
+        const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
+        if (key < worst) heap.add(key, j, worst);
+    };
+
+    // per-lane scan of sorted positions [s, e) straight from global memory,
+    // kBatch loads in flight
+    auto scan_direct = [&](uint32_t s, uint32_t e) {
+        for (uint32_t j = s; j < e; j += kBatch)
+        {
+            float4 p[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                if (j + u < e) p[u] = pts[j + u];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                if (j + u < e) consider(p[u], j + u);
+        }
+    };
+
+    // this lane's part [s, e) of the staged chunk [c, c + cnt), kBatch LDS
+    // reads in flight
+    auto scan_staged = [&](uint32_t s, uint32_t e, uint32_t c, uint32_t cnt) {
+        const uint32_t lo = max(s, c), hi = min(e, c + cnt);
+        for (uint32_t j = lo; j < hi; j += kBatch)
+        {
+            float4 p[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                if (j + u < hi) p[u] = stage[j + u - c];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                if (j + u < hi) consider(p[u], j + u);
+        }
+    };
+
+    // this lane's x pieces of row (y, z) of box B minus box C
+    auto row_pieces = [&](const Box& B, const Box& C, bool hasC, bool tighten, double dz2, int y, int z, int& a0,
+                          int& b0, int& a1, int& b1) {
+        a0 = 1, b0 = 0, a1 = 1, b1 = 0;
+        int xa = B.x0, xb = B.x1;
+        if (tighten)
+        {
+            const double wl = wlim();
+            const double dy = axis_gap(qy, y, G.lo[1], h, margin);
+            const double rem = wl - dz2 - dy * dy;
+            if (rem < 0.0) return;
+            if (rem < INFINITY)
+            {
+                const double xr = sqrt(rem) + margin;
+                xa = max(xa, cell_coord(qx - xr, G.lo[0], G.inv_h, G0));
+                xb = min(xb, cell_coord(qx + xr, G.lo[0], G.inv_h, G0));
+            }
+        }
+        if (xa > xb) return;
+        if (hasC && y >= C.y0 && y <= C.y1 && z >= C.z0 && z <= C.z1)
+        {
+            a0 = xa, b0 = min(xb, C.x0 - 1);
+            a1 = max(xa, C.x1 + 1), b1 = xb;
+        }
+        else
+            a0 = xa, b0 = xb;
+    };
+
+    // Scan the cells of box B minus box C (per lane; empty boxes allowed).
+    // With `tighten`, each row's x-range is cut to the ball of the lane's
+    // current worst distance.
+    auto scan_pass = [&](const Box& B, const Box& C, bool tighten) {
+        const bool has = B.x0 <= B.x1 && B.y0 <= B.y1 && B.z0 <= B.z1;
+        const bool hasC = C.x0 <= C.x1 && C.y0 <= C.y1 && C.z0 <= C.z1;
+        const int Z0 = wave_min_i(has ? B.z0 : INT_MAX), Z1 = wave_max_i(has ? B.z1 : INT_MIN);
+        const int Y0 = wave_min_i(has ? B.y0 : INT_MAX), Y1 = wave_max_i(has ? B.y1 : INT_MIN);
+        if (Z0 > Z1) return;  // no lane has work
+        const int lmin = wave_min_i(has ? level : INT_MAX), lmax = wave_max_i(has ? level : INT_MIN);
+        const int my_rows = has ? (B.z1 - B.z0 + 1) * (B.y1 - B.y0 + 1) : 0;
+        const int max_rows = wave_max_i(my_rows);
+        const long union_rows = (long)(Z1 - Z0 + 1) * (long)(Y1 - Y0 + 1);
+        const bool coop = (lmin == lmax) && union_rows <= 2L * max_rows + 8;
+        if (coop)
+        {
+            // every lane stages from the working level's array, also lanes
+            // that sit on another level and have no row in this pass
+            const float4* __restrict__ wpts = pick_level(a, lmin).pts;
+            for (int z = Z0; z <= Z1; ++z)
+            {
+                const bool zin = has && z >= B.z0 && z <= B.z1;
+                double dz2 = 0.0;
+                if (tighten && zin)
+                {
+                    const double dz = axis_gap(qz, z, G.lo[2], h, margin);
+                    dz2 = dz * dz;
+                }
+                for (int y = Y0; y <= Y1; ++y)
+                {
+                    int a0 = 1, b0 = 0, a1 = 1, b1 = 0;
+                    if (zin && y >= B.y0 && y <= B.y1) row_pieces(B, C, hasC, tighten, dz2, y, z, a0, b0, a1, b1);
+                    const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
+                    uint32_t s0 = 0xFFFFFFFFu, e0 = 0, s1 = 0xFFFFFFFFu, e1 = 0;
+                    if (a0 <= b0)
                     {
-                        const bool ye = (y == cy - n) || (y == cy + n);
-                        if (ze || ye)
-                            row(y, z, x0, x1);
-                        else
+                        s0 = cs[base + (uint32_t)a0];
+                        e0 = cs[base + (uint32_t)b0 + 1];
+                    }
+                    if (a1 <= b1)
+                    {
+                        s1 = cs[base + (uint32_t)a1];
+                        e1 = cs[base + (uint32_t)b1 + 1];
+                    }
+                    const uint32_t S = wave_min_u(min(s0 < e0 ? s0 : 0xFFFFFFFFu, s1 < e1 ? s1 : 0xFFFFFFFFu));
+                    const uint32_t E = wave_max_u(max(s0 < e0 ? e0 : 0u, s1 < e1 ? e1 : 0u));
+                    for (uint32_t c = S; c < E; c += kStage)
+                    {
+                        const uint32_t cnt = min((uint32_t)kStage, E - c);
+#pragma unroll
+                        for (int u = 0; u < kStage / 64; ++u)
                         {
-                            if (cx - n >= 0) row(y, z, cx - n, cx - n);
-                            if (cx + n < G0) row(y, z, cx + n, cx + n);
+                            const uint32_t i = (uint32_t)(u * 64 + lane);
+                            if (i < cnt) stage[i] = wpts[c + i];
                         }
+                        wave_sync();
+                        scan_staged(s0, e0, c, cnt);
+                        scan_staged(s1, e1, c, cnt);
+                        wave_sync();
                     }
                 }
+            }
+        }
+        else if (has)
+        {
+            for (int z = B.z0; z <= B.z1; ++z)
+            {
+                double dz2 = 0.0;
+                if (tighten)
+                {
+                    const double dz = axis_gap(qz, z, G.lo[2], h, margin);
+                    dz2 = dz * dz;
+                    if (dz2 > wlim()) continue;
+                }
+                for (int y = B.y0; y <= B.y1; ++y)
+                {
+                    int a0, b0, a1, b1;
+                    row_pieces(B, C, hasC, tighten, dz2, y, z, a0, b0, a1, b1);
+                    const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
+                    uint32_t s0 = 0, e0 = 0, s1 = 0, e1 = 0;
+                    if (a0 <= b0)
+                    {
+                        s0 = cs[base + (uint32_t)a0];
+                        e0 = cs[base + (uint32_t)b0 + 1];
+                    }
+                    if (a1 <= b1)
+                    {
+                        s1 = cs[base + (uint32_t)a1];
+                        e1 = cs[base + (uint32_t)b1 + 1];
+                    }
+                    scan_direct(s0, e0);
+                    scan_direct(s1, e1);
+                }
+            }
+        }
+    };
+
+    // Phase 1 (kNN): cubic shells around the query cell until the heap holds
+    // k candidates.
+    int n_done = -1;
+    bool covered = false;
+    if (!RADIUS)
+    {
+        bool need = live;
+        for (int n = BUDGET ? 1 : 0; __ballot(need) != 0; ++n)
+        {
+            const Box B = need ? cube_box(cx, cy, cz, n, G) : empty_box();
+            const Box C = (need && n > (BUDGET ? 1 : 0)) ? cube_box(cx, cy, cz, n - 1, G) : empty_box();
+            scan_pass(B, C, false);
+            if (need)
+            {
                 n_done = n;
                 covered = (cx - n <= 0) && (cx + n >= G0 - 1) && (cy - n <= 0) && (cy + n >= G1 - 1) &&
                           (cz - n <= 0) && (cz + n >= G2 - 1);
-                if (worst != sentinel || covered) break;
-            }
-        }
-
-        // Phase 2: every cell row that intersects the ball of the current
-        // k-th distance (or of the search radius), minus the phase-1 cube.
-        if (!covered)
-        {
-            const double h = gp.h, margin = (double)a.margin;
-            auto wlim = [&]() -> double {
-                if (worst == ~0ull) return INFINITY;
-                const float w = __uint_as_float((uint32_t)(worst >> 32));
-                if (!(w < INFINITY)) return INFINITY;
-                return (double)w * 1.00001 + 1e-37;
-            };
-            double wl = wlim();
-            int tx0 = 0, tx1 = G0 - 1, ty0 = 0, ty1 = G1 - 1, tz0 = 0, tz1 = G2 - 1;
-            if (wl < INFINITY)
-            {
-                const double R = sqrt(wl) + margin;
-                tx0 = cell_coord(qx - R, gp.lo[0], gp.inv_h, G0);
-                tx1 = cell_coord(qx + R, gp.lo[0], gp.inv_h, G0);
-                ty0 = cell_coord(qy - R, gp.lo[1], gp.inv_h, G1);
-                ty1 = cell_coord(qy + R, gp.lo[1], gp.inv_h, G1);
-                tz0 = cell_coord(qz - R, gp.lo[2], gp.inv_h, G2);
-                tz1 = cell_coord(qz + R, gp.lo[2], gp.inv_h, G2);
-            }
-            const int ix0 = n_done >= 0 ? max(cx - n_done, 0) : 1, ix1 = n_done >= 0 ? min(cx + n_done, G0 - 1) : 0;
-            for (int z = tz0; z <= tz1; ++z)
-            {
-                const double dz = axis_gap(qz, z, gp.lo[2], h, margin);
-                wl = wlim();
-                if (dz * dz > wl) continue;
-                const bool zin = n_done >= 0 && abs(z - cz) <= n_done;
-                for (int y = ty0; y <= ty1; ++y)
+                if (worst != sentinel || covered) need = false;
+                if (BUDGET && need && n >= 1)
                 {
-                    const double dy = axis_gap(qy, y, gp.lo[1], h, margin);
-                    wl = wlim();
-                    const double rem = wl - dz * dz - dy * dy;
-                    if (rem < 0.0) continue;
-                    int xa = tx0, xb = tx1;
-                    if (rem < INFINITY)
-                    {
-                        const double xr = sqrt(rem) + margin;
-                        xa = max(xa, cell_coord(qx - xr, gp.lo[0], gp.inv_h, G0));
-                        xb = min(xb, cell_coord(qx + xr, gp.lo[0], gp.inv_h, G0));
-                    }
-                    if (xa > xb) continue;
-                    if (zin && abs(y - cy) <= n_done)
-                    {
-                        if (xa < ix0) row(y, z, xa, min(xb, ix0 - 1));
-                        if (xb > ix1) row(y, z, max(xa, ix1 + 1), xb);
-                    }
-                    else
-                        row(y, z, xa, xb);
+                    // sparse neighbourhood: leave it to the multi-level pass
+                    need = false;
+                    deferred = true;
+                    live = false;
                 }
             }
         }
     }
 
-    heap.sort_ascending();
+    // Phase 2: every row meeting the ball of the current k-th distance (kNN)
+    // or of the search radius, minus the phase-1 cube.
+    {
+        Box B = empty_box();
+        if (live && !covered)
+        {
+            const double wl = wlim();
+            if (wl < INFINITY)
+            {
+                const double R = sqrt(wl) + margin;
+                B = Box{cell_coord(qx - R, G.lo[0], G.inv_h, G0), cell_coord(qx + R, G.lo[0], G.inv_h, G0),
+                        cell_coord(qy - R, G.lo[1], G.inv_h, G1), cell_coord(qy + R, G.lo[1], G.inv_h, G1),
+                        cell_coord(qz - R, G.lo[2], G.inv_h, G2), cell_coord(qz + R, G.lo[2], G.inv_h, G2)};
+            }
+            else
+                B = Box{0, G0 - 1, 0, G1 - 1, 0, G2 - 1};
+            if (BUDGET && (B.x0 < cx - 2 || B.x1 > cx + 2 || B.y0 < cy - 2 || B.y1 > cy + 2 || B.z0 < cz - 2 ||
+                           B.z1 > cz + 2))
+            {
+                B = empty_box();
+                deferred = true;
+                live = false;
+            }
+        }
+        const Box C = n_done >= 0 ? cube_box(cx, cy, cz, n_done, G) : empty_box();
+        scan_pass(B, C, true);
+    }
+
+    if (!active) return;
+    if (BUDGET)
+    {
+        a.defer_flag[t] = deferred ? 1u : 0u;
+        if (deferred) return;
+    }
+    heap.finish(sentinel);
     const uint32_t k = a.k;
 
     if (MODE == kModeKnn)
@@ -547,7 +780,7 @@ __global__ __launch_bounds__(BLOCK) void k_search(SearchArgs a)
         uint32_t* oi = reinterpret_cast<uint32_t*>(a.out_idx) + (size_t)qorig * k;
         float* od = a.out_d2 ? a.out_d2 + (size_t)qorig * k : nullptr;
         float f[11];
-        if (finite)
+        if (live)
         {
             Moments m;
             mom_zero(m);
@@ -582,26 +815,335 @@ __global__ __launch_bounds__(BLOCK) void k_search(SearchArgs a)
     }
 }
 
-template <int MODE, int BLOCK>
-hipError_t launch_search_mb(const SearchArgs& a, size_t lds, bool gh, hipStream_t s)
+// ----------------------------------------------------------- selected ----
+// compute_features_selected (include/pgeof.hpp:325-375): radius neighbourhood
+// in the input precision T, the min(found, max_knn) nearest hits (ties by
+// index), rows with fewer than 2 hits stay 0, then the EFeatureID list of
+// include/pca.hpp:212-295 evaluated in T.
+template <typename T>
+struct PairHeap
 {
-    const uint32_t nb = (a.n_query + BLOCK - 1) / BLOCK;
+    T* key;
+    uint32_t* idx;
+    uint32_t stride;
+    uint32_t k;
+    __device__ __forceinline__ static bool less(T a, uint32_t ia, T b, uint32_t ib)
+    {
+        return a < b || (a == b && ia < ib);
+    }
+    __device__ __forceinline__ void sift_root(T kk, uint32_t ii, uint32_t n)
+    {
+        uint32_t i = 0;
+        for (;;)
+        {
+            const uint32_t l = 2 * i + 1;
+            if (l >= n) break;
+            T ck = key[l * stride];
+            uint32_t ci = idx[l * stride];
+            uint32_t c = l;
+            if (l + 1 < n)
+            {
+                const T rk = key[(l + 1) * stride];
+                const uint32_t ri = idx[(l + 1) * stride];
+                if (less(ck, ci, rk, ri))
+                {
+                    ck = rk;
+                    ci = ri;
+                    c = l + 1;
+                }
+            }
+            if (!less(kk, ii, ck, ci)) break;
+            key[i * stride] = ck;
+            idx[i * stride] = ci;
+            i = c;
+        }
+        key[i * stride] = kk;
+        idx[i * stride] = ii;
+    }
+    __device__ __forceinline__ void sort_ascending()
+    {
+        for (uint32_t end = k - 1; end > 0; --end)
+        {
+            const T tk = key[0];
+            const uint32_t ti = idx[0];
+            const T lk = key[end * stride];
+            const uint32_t li = idx[end * stride];
+            key[end * stride] = tk;
+            idx[end * stride] = ti;
+            sift_root(lk, li, end);
+        }
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T sqdist_t(const T* q, const T* p)
+{
+#pragma clang fp contract(off)
+    const T dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    T d = dx * dx;
+    d = d + dy * dy;
+    d = d + dz * dz;
+    return d;
+}
+
+template <typename T>
+struct PcaT
+{
+    T val[3];
+    T v[3][3];
+};
+
+template <typename T>
+__device__ __forceinline__ PcaT<T> to_pca_t(const Eig& E)
+{
+    PcaT<T> P;
+    for (int i = 0; i < 3; ++i)
+    {
+        P.val[i] = (T)E.l[i] > (T)0 ? (T)E.l[i] : (T)0;
+        for (int j = 0; j < 3; ++j) P.v[i][j] = (T)E.v[i][j];
+    }
+    bool flip = P.v[2][2] < (T)0;
+    if (P.v[2][2] == (T)0) flip = (P.v[2][1] < (T)0) || (P.v[2][1] == (T)0 && P.v[2][0] < (T)0);
+    if (flip)
+        for (int j = 0; j < 3; ++j) P.v[2][j] = -P.v[2][j];
+    return P;
+}
+
+template <typename T>
+__device__ __forceinline__ T selected_feature(const PcaT<T>& P, int fid)
+{
+#pragma clang fp contract(off)
+    const T eps = (T)1e-3;
+    const T val0 = sqrt(P.val[0]), val1 = sqrt(P.val[1]), val2 = sqrt(P.val[2]);
+    const T fact = (T)1.0 / (val0 + eps);
+    switch (fid)
+    {
+        case 0: return (val0 - val1) * fact;
+        case 1: return (val1 - val2) * fact;
+        case 2: return val2 * fact;
+        case 3:
+            if (val0 > (T)0)
+            {
+                T u[3];
+                for (int c = 0; c < 3; ++c)
+                    u[c] = P.val[0] * fabs(P.v[0][c]) + P.val[1] * fabs(P.v[1][c]) + P.val[2] * fabs(P.v[2][c]);
+                return u[2] / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            }
+            return (T)0;
+        case 4: return P.v[2][0];
+        case 5: return P.v[2][1];
+        case 6: return P.v[2][2];
+        case 7: return val0;
+        case 8: return sqrt(val0 * val1 + (T)1e-6f);  // pca.hpp:252: float literal in both builds
+        case 9: return pow(val0 * val1 * val2 + (T)1e-9, (T)1.0 / (T)3.0);
+        case 10: return val2 / (val0 + val1 + val2 + eps);
+        case 12: return (T)1.0 - fabs(P.v[2][2]);
+        case 13:
+        {
+            const T sum = P.val[0] + P.val[1] + P.val[2] + eps;
+            const T e0 = P.val[0] / sum, e1 = P.val[1] / sum, e2 = P.val[2] / sum;
+            return -e0 * log(e0 + eps) - e1 * log(e1 + eps) - e2 * log(e2 + eps);
+        }
+        default: return (T)0;
+    }
+}
+
+template <typename T, int BLOCK, bool GLOBAL_HEAP>
+__global__ __launch_bounds__(BLOCK) void k_selected(SelectedArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const uint32_t t = lb * BLOCK + threadIdx.x;
+    if (t >= a.n) return;
+    const uint32_t cap = a.cap;
+    PairHeap<T> heap;
+    if (GLOBAL_HEAP)
+    {
+        heap.key = reinterpret_cast<T*>(a.gheap_keys) + t;
+        heap.idx = a.gheap_idx + t;
+        heap.stride = a.n;
+    }
+    else
+    {
+        heap.key = reinterpret_cast<T*>(smem) + threadIdx.x;
+        heap.idx = reinterpret_cast<uint32_t*>(smem + (size_t)cap * BLOCK * sizeof(T)) + threadIdx.x;
+        heap.stride = BLOCK;
+    }
+    heap.k = cap;
+    const T r2 = (T)a.r2;
+    for (uint32_t j = 0; j < cap; ++j)
+    {
+        heap.key[j * heap.stride] = r2;
+        heap.idx[j * heap.stride] = 0xFFFFFFFFu;
+    }
+    const T* __restrict__ X = reinterpret_cast<const T*>(a.xyz);
+    const float4 qf = a.grid.pts[t];
+    const uint32_t qi = __float_as_uint(qf.w);
+    const T q[3] = {X[(size_t)qi * 3 + 0], X[(size_t)qi * 3 + 1], X[(size_t)qi * 3 + 2]};
+    T worst_k = r2;
+    uint32_t worst_i = 0xFFFFFFFFu;
+    const GridParams& gp = a.grid.p;
+    const bool finite = isfinite((double)q[0]) && isfinite((double)q[1]) && isfinite((double)q[2]);
+    if (finite)
+    {
+        const double R = a.reach;
+        const int x0 = cell_coord((double)qf.x - R, gp.lo[0], gp.inv_h, gp.g[0]);
+        const int x1 = cell_coord((double)qf.x + R, gp.lo[0], gp.inv_h, gp.g[0]);
+        const int y0 = cell_coord((double)qf.y - R, gp.lo[1], gp.inv_h, gp.g[1]);
+        const int y1 = cell_coord((double)qf.y + R, gp.lo[1], gp.inv_h, gp.g[1]);
+        const int z0 = cell_coord((double)qf.z - R, gp.lo[2], gp.inv_h, gp.g[2]);
+        const int z1 = cell_coord((double)qf.z + R, gp.lo[2], gp.inv_h, gp.g[2]);
+        for (int z = z0; z <= z1; ++z)
+            for (int y = y0; y <= y1; ++y)
+            {
+                const uint32_t base = ((uint32_t)z * (uint32_t)gp.g[1] + (uint32_t)y) * (uint32_t)gp.g[0];
+                const uint32_t s = a.grid.cell_start[base + (uint32_t)x0], e = a.grid.cell_start[base + (uint32_t)x1 + 1];
+                for (uint32_t j = s; j < e; ++j)
+                {
+                    const uint32_t pi = __float_as_uint(a.grid.pts[j].w);
+                    const T p[3] = {X[(size_t)pi * 3 + 0], X[(size_t)pi * 3 + 1], X[(size_t)pi * 3 + 2]};
+                    const T d = sqdist_t<T>(q, p);
+                    if (d < r2 && PairHeap<T>::less(d, pi, worst_k, worst_i))
+                    {
+                        heap.sift_root(d, pi, cap);
+                        worst_k = heap.key[0];
+                        worst_i = heap.idx[0];
+                    }
+                }
+            }
+    }
+    heap.sort_ascending();
+    uint32_t cnt = 0;
+    for (uint32_t j = 0; j < cap; ++j)
+        if (heap.idx[j * heap.stride] != 0xFFFFFFFFu) ++cnt;
+    T* out = reinterpret_cast<T*>(a.out) + (size_t)qi * a.n_sel;
+    if (cnt < 2 || a.max_knn == 0)
+    {
+        for (uint32_t s = 0; s < a.n_sel; ++s) out[s] = (T)0;
+        return;
+    }
+    const uint32_t m = cnt < a.max_knn ? cnt : a.max_knn;
+    Moments mo;
+    mom_zero(mo);
+    double o[3] = {0.0, 0.0, 0.0};
+    for (uint32_t j = 0; j < m; ++j)
+    {
+        const uint32_t pi = heap.idx[j * heap.stride];
+        const double px = (double)X[(size_t)pi * 3 + 0], py = (double)X[(size_t)pi * 3 + 1],
+                     pz = (double)X[(size_t)pi * 3 + 2];
+        if (j == 0) o[0] = px, o[1] = py, o[2] = pz;
+        mom_add(mo, px - o[0], py - o[1], pz - o[2]);
+    }
+    double c[6];
+    mom_cov(mo, (double)m, c);
+    const Eig E = eig_full(c);
+    const PcaT<T> P = to_pca_t<T>(E);
+    for (uint32_t s = 0; s < a.n_sel; ++s) out[s] = selected_feature<T>(P, a.fids[s]);
+}
+
+template <typename T, int BLOCK>
+hipError_t launch_selected_b(const SelectedArgs& a, size_t lds, bool gh, hipStream_t s)
+{
+    const uint32_t nb = (a.n + BLOCK - 1) / BLOCK;
     if (nb == 0) return hipSuccess;
     if (gh)
-        k_search<MODE, BLOCK, true><<<nb, BLOCK, 0, s>>>(a);
+        k_selected<T, BLOCK, true><<<nb, BLOCK, 0, s>>>(a);
     else
-        k_search<MODE, BLOCK, false><<<nb, BLOCK, lds, s>>>(a);
+        k_selected<T, BLOCK, false><<<nb, BLOCK, lds, s>>>(a);
     return hipGetLastError();
 }
 
-template <int MODE>
+template <typename T>
+hipError_t launch_selected_t(const SelectedArgs& a, int block, size_t lds, bool gh, hipStream_t s)
+{
+    switch (block)
+    {
+        case 64: return launch_selected_b<T, 64>(a, lds, gh, s);
+        case 128: return launch_selected_b<T, 128>(a, lds, gh, s);
+        default: return launch_selected_b<T, 256>(a, lds, gh, s);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        out[i] = (float)in[i];
+}
+
+template <int MODE, int WAVES, bool BUDGET>
+hipError_t launch_search_mb(const SearchArgs& a, size_t lds, bool gh, hipStream_t s)
+{
+    constexpr int BLOCK = WAVES * 64;
+    const uint32_t nb = (a.n_query + BLOCK - 1) / BLOCK;
+    if (nb == 0) return hipSuccess;
+    if (gh)
+        k_search<MODE, WAVES, true, BUDGET><<<nb, BLOCK, (size_t)WAVES * kStage * sizeof(float4), s>>>(a);
+    else
+        k_search<MODE, WAVES, false, BUDGET><<<nb, BLOCK, lds, s>>>(a);
+    return hipGetLastError();
+}
+
+template <int MODE, bool BUDGET>
 hipError_t launch_search_m(const SearchArgs& a, int block, size_t lds, bool gh, hipStream_t s)
 {
     switch (block)
     {
-        case 64: return launch_search_mb<MODE, 64>(a, lds, gh, s);
-        case 128: return launch_search_mb<MODE, 128>(a, lds, gh, s);
-        default: return launch_search_mb<MODE, 256>(a, lds, gh, s);
+        case 64: return launch_search_mb<MODE, 1, BUDGET>(a, lds, gh, s);
+        case 128: return launch_search_mb<MODE, 2, BUDGET>(a, lds, gh, s);
+        default: return launch_search_mb<MODE, 4, BUDGET>(a, lds, gh, s);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact_queries(const float4* __restrict__ q, uint32_t n,
+                                                         const uint32_t* __restrict__ flag,
+                                                         const uint32_t* __restrict__ off, float4* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n && flag[i]) out[off[i]] = q[i];
+}
+
+// Histogram for grids with few cells: per-block counts in LDS, one global
+// atomic per touched cell and block; rank = block base + rank inside the block.
+constexpr uint32_t kSmallHistCells = 8192;
+__global__ __launch_bounds__(256) void k_hist_small(const float* __restrict__ xyz, uint32_t n, GridParams gp,
+                                                    uint32_t* __restrict__ count, uint32_t* __restrict__ cell,
+                                                    uint32_t* __restrict__ rank)
+{
+    __shared__ uint32_t bins[kSmallHistCells];
+    const uint32_t G = (uint32_t)gp.G;
+    for (uint32_t c = threadIdx.x; c < G; c += 256) bins[c] = 0;
+    __syncthreads();
+    constexpr uint32_t kPer = 16;  // points per thread
+    const uint64_t base = (uint64_t)blockIdx.x * 256 * kPer;
+    uint32_t cid[kPer], lr[kPer];
+    for (uint32_t u = 0; u < kPer; ++u)
+    {
+        const uint64_t i = base + (uint64_t)u * 256 + threadIdx.x;
+        cid[u] = 0xFFFFFFFFu;
+        if (i < n)
+        {
+            const int cx = cell_coord((double)xyz[i * 3 + 0], gp.lo[0], gp.inv_h, gp.g[0]);
+            const int cy = cell_coord((double)xyz[i * 3 + 1], gp.lo[1], gp.inv_h, gp.g[1]);
+            const int cz = cell_coord((double)xyz[i * 3 + 2], gp.lo[2], gp.inv_h, gp.g[2]);
+            cid[u] = ((uint32_t)cz * (uint32_t)gp.g[1] + (uint32_t)cy) * (uint32_t)gp.g[0] + (uint32_t)cx;
+            lr[u] = atomicAdd(&bins[cid[u]], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < G; c += 256)
+    {
+        const uint32_t v = bins[c];
+        bins[c] = v ? atomicAdd(&count[c], v) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t u = 0; u < kPer; ++u)
+    {
+        const uint64_t i = base + (uint64_t)u * 256 + threadIdx.x;
+        if (i < n)
+        {
+            if (cell) cell[i] = cid[u];
+            if (rank) rank[i] = bins[cid[u]] + lr[u];
+        }
     }
 }
 
@@ -641,7 +1183,18 @@ hipError_t launch_hist(const float* xyz, uint32_t n, const GridParams& gp, uint3
 {
     if (n == 0) return hipSuccess;
     ProfScope ps("grid_histogram", s);
-    k_hist<<<(n + 255) / 256, 256, 0, s>>>(xyz, n, gp, count, cell, rank);
+    if (gp.G <= kSmallHistCells)
+        k_hist_small<<<(n + 256 * 16 - 1) / (256 * 16), 256, 0, s>>>(xyz, n, gp, count, cell, rank);
+    else
+        k_hist<<<(n + 255) / 256, 256, 0, s>>>(xyz, n, gp, count, cell, rank);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_queries(const float4* q, uint32_t n, const uint32_t* flag, const uint32_t* off,
+                                  float4* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_compact_queries<<<(n + 255) / 256, 256, 0, s>>>(q, n, flag, off, out);
     return hipGetLastError();
 }
 
@@ -683,16 +1236,40 @@ hipError_t launch_compact_rows(const uint32_t* dense, uint32_t n_rows, uint32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds, bool gh, hipStream_t s)
+hipError_t launch_selected(bool f64, const SelectedArgs& a, int block, size_t lds, bool gh, hipStream_t s)
+{
+    ProfScope ps(f64 ? "features_selected_f64" : "features_selected_f32", s);
+    return f64 ? launch_selected_t<double>(a, block, lds, gh, s) : launch_selected_t<float>(a, block, lds, gh, s);
+}
+
+hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_f64_to_f32<<<blocks_for(n, 256, 65536), 256, 0, s>>>(in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds, bool gh, bool budget, hipStream_t s)
 {
     static const char* names[4] = {"knn_search", "radius_search", "knn_features_fused", "radius_search_csr"};
-    ProfScope ps(names[mode & 3], s);
+    static const char* names_b[4] = {"knn_search_dense", "radius_search", "knn_features_fused_dense",
+                                     "radius_search_csr"};
+    ProfScope ps(budget ? names_b[mode & 3] : names[mode & 3], s);
+    if (budget)
+    {
+        switch (mode)
+        {
+            case kModeKnn: return launch_search_m<kModeKnn, true>(a, block, lds, gh, s);
+            case kModeKnnFeat: return launch_search_m<kModeKnnFeat, true>(a, block, lds, gh, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (mode)
     {
-        case kModeKnn: return launch_search_m<kModeKnn>(a, block, lds, gh, s);
-        case kModeRadius: return launch_search_m<kModeRadius>(a, block, lds, gh, s);
-        case kModeKnnFeat: return launch_search_m<kModeKnnFeat>(a, block, lds, gh, s);
-        default: return launch_search_m<kModeRadiusCsr>(a, block, lds, gh, s);
+        case kModeKnn: return launch_search_m<kModeKnn, false>(a, block, lds, gh, s);
+        case kModeRadius: return launch_search_m<kModeRadius, false>(a, block, lds, gh, s);
+        case kModeKnnFeat: return launch_search_m<kModeKnnFeat, false>(a, block, lds, gh, s);
+        default: return launch_search_m<kModeRadiusCsr, false>(a, block, lds, gh, s);
     }
 }
 

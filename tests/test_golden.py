@@ -1,0 +1,55 @@
+"""Golden vectors (tests/golden/pgeof_golden.npz, made by make_golden.py):
+the oracle must reproduce them exactly on this machine (CPU), and the GPU path
+must match them (bit-exact neighbours, features within tests/parity.py)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.parity import assert_features_close
+
+G = np.load(os.path.join(os.path.dirname(__file__), "golden", "pgeof_golden.npz"), allow_pickle=False)
+
+
+def test_oracle_reproduces_golden():
+    xyz, k = G["xyz"], int(G["k"])
+    idx, d2 = O.knn(xyz, xyz, k, kdtree=True)
+    np.testing.assert_array_equal(idx, G["knn_idx"])
+    np.testing.assert_array_equal(d2, G["knn_d2"])
+    np.testing.assert_allclose(O.compute_features(xyz, G["nn"], G["nn_ptr"]), G["features"], rtol=1e-12,
+                               atol=1e-12)
+    np.testing.assert_allclose(O.compute_features_multiscale(xyz, G["nn"], G["nn_ptr"], G["scales"]),
+                               G["multiscale"], rtol=1e-12, atol=1e-12)
+    a = G["optimal_args"]
+    np.testing.assert_allclose(O.compute_features_optimal(xyz, G["nn"], G["nn_ptr"], *a), G["optimal"],
+                               rtol=1e-12, atol=1e-12)
+    ri, rd, _ = O.radius(xyz, xyz, float(G["radius"]), int(G["max_knn"]), kdtree=True)
+    np.testing.assert_array_equal(ri, G["radius_idx"])
+    np.testing.assert_array_equal(rd, G["radius_d2"])
+
+
+@pytest.mark.gpu
+def test_gpu_matches_golden():
+    import point_geometric_features_amd as pgeof
+
+    xyz, k = G["xyz"], int(G["k"])
+    idx, d2 = pgeof.knn_search(xyz, xyz, k)
+    np.testing.assert_array_equal(idx, G["knn_idx"])
+    np.testing.assert_array_equal(d2, G["knn_d2"])
+    nn, ptr = G["nn"], G["nn_ptr"]
+    val = O.eigvals_csr_np(xyz, nn, ptr)
+    assert_features_close(pgeof.compute_features(xyz, nn, ptr), G["features"], val, "golden features")
+    m = pgeof.compute_features_multiscale(xyz, nn, ptr, [int(s) for s in G["scales"]])
+    for s, kk in enumerate(G["scales"]):
+        assert_features_close(m[:, s], G["multiscale"][:, s], O.eigvals_csr_np(xyz, nn, ptr, prefix=int(kk)),
+                              f"golden multiscale {kk}")
+    a = [int(v) for v in G["optimal_args"]]
+    o = pgeof.compute_features_optimal(xyz, nn, ptr, *a)
+    assert (o[:, 11] == G["optimal"][:, 11]).mean() > 0.99
+    ri, rd = pgeof.radius_search(xyz, xyz, float(G["radius"]), int(G["max_knn"]))
+    np.testing.assert_array_equal(ri, G["radius_idx"])
+    np.testing.assert_array_equal(rd, G["radius_d2"])
+    rf = pgeof.compute_features(xyz, G["radius_nn"], G["radius_nn_ptr"])
+    assert_features_close(rf, G["radius_features"], O.eigvals_csr_np(xyz, G["radius_nn"], G["radius_nn_ptr"]),
+                          "golden radius features")

@@ -139,3 +139,30 @@ def test_radius_errors():
     xyz = scenes.uniform_cube(50, seed=15)
     with pytest.raises(ValueError):
         pgeof.radius_search(xyz, xyz, 0.1, 51)
+
+
+def test_knn_sparse_outliers_use_coarse_levels():
+    # a dense cluster plus isolated far points: the isolated queries need the
+    # coarse grid levels, the dense ones the fine level
+    rng = np.random.default_rng(16)
+    dense = rng.normal(0, 0.05, size=(20_000, 3))
+    sparse = rng.uniform(-50, 50, size=(300, 3))
+    xyz = np.concatenate([dense, sparse]).astype(np.float32)
+    xyz = xyz[rng.permutation(len(xyz))]
+    _eq_knn(xyz, xyz, 20, kdtree=False)
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_knn_level_count_does_not_change_results(levels):
+    from point_geometric_features_amd import capi
+
+    xyz = scenes.lidar_scene(60_000, seed=17, extent=25.0)
+    old = capi.get_tuning("grid_levels")
+    try:
+        capi.set_tuning("grid_levels", levels)
+        gi, gd = pgeof.knn_search(xyz, xyz, 20)
+    finally:
+        capi.set_tuning("grid_levels", old)
+    oi, od = O.knn(xyz, xyz, 20, kdtree=True)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)

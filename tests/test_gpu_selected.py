@@ -1,0 +1,74 @@
+"""GPU parity of compute_features_selected (include/pgeof.hpp:325-375,
+include/pca.hpp:212-295) against the oracle's brute-force restatement, for
+the float32 and the float64 overload (src/pgeof_ext.cpp:148-177)."""
+import numpy as np
+import pytest
+
+import point_geometric_features_amd as pgeof
+from oracle import oracle as O
+from point_geometric_features_amd import EFeatureID as F
+
+pytestmark = pytest.mark.gpu
+
+ALL = [F.Linearity, F.Planarity, F.Scattering, F.VerticalityPGEOF, F.Normal_x, F.Normal_y, F.Normal_z, F.Length,
+       F.Surface, F.Volume, F.Curvature, F.K_optimal, F.Verticality, F.Eigentropy]
+
+
+def _vals_selected(xyz, r, max_knn):
+    """FP64 eigenvalues of each point's selected neighbourhood (for the unpinned masks)."""
+    idx, _, cnt = O.radius(xyz.astype(np.float32), xyz.astype(np.float32), r, max(max_knn, 1), kdtree=False)
+    nn = idx[idx >= 0].astype(np.uint32)
+    ptr = np.r_[0, (idx >= 0).sum(1).cumsum()].astype(np.uint32)
+    return O.eigvals_csr_np(xyz.astype(np.float32), nn, ptr)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_selected_all_features_vs_oracle(dtype):
+    rng = np.random.default_rng(51)
+    xyz = (rng.random((3000, 3)) * [2.0, 2.0, 0.5]).astype(dtype)
+    r, k = 0.15, 24
+    got = pgeof.compute_features_selected(xyz, r, k, ALL)
+    assert got.dtype == dtype and got.shape == (3000, len(ALL))
+    ref = O.compute_features_selected(xyz, r, k, ALL)
+    val = _vals_selected(xyz, r, k)
+    rankdef = val[:, 0] <= 1e-9 * np.maximum(val[:, 2], 1e-300)
+    top = (val[:, 2] - val[:, 1]) <= 1e-6 * np.maximum(val[:, 2], 1e-300)
+    tol_rel = 1e-5 if dtype == np.float32 else 1e-9
+    atol = 1e-4 if dtype == np.float32 else 1e-8
+    for j, fid in enumerate(ALL):
+        err = np.abs(got[:, j].astype(np.float64) - ref[:, j])
+        tol = atol + tol_rel * np.abs(ref[:, j])
+        skip = np.zeros(len(xyz), bool)
+        if fid in (F.VerticalityPGEOF,):
+            skip |= rankdef | top
+        if fid in (F.Normal_x, F.Normal_y, F.Normal_z, F.Verticality):
+            skip |= top
+        if fid in (F.Linearity, F.Planarity, F.Scattering, F.Volume):
+            # rank-deficient rows (e.g. 3 points): sqrt(lambda_min) of a
+            # rounding-level lambda_min enters through epsilon = 1e-3
+            tol[rankdef] = atol + 1e-3 * np.abs(ref[rankdef, j])
+        if fid == F.Length:
+            tol[rankdef] = 1e-6
+        bad = (err > tol) & ~skip
+        assert not bad.any(), (fid, np.nonzero(bad)[0][:5], got[bad, j][:5], ref[bad, j][:5])
+
+
+def test_selected_sparse_rows_and_caps():
+    rng = np.random.default_rng(52)
+    xyz = rng.random((2000, 3), dtype=np.float32)
+    # radius so small that many points have < 2 neighbours -> zero rows
+    got = pgeof.compute_features_selected(xyz, 0.03, 8, [F.Verticality, F.Curvature])
+    ref = O.compute_features_selected(xyz, 0.03, 8, [F.Verticality, F.Curvature])
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-4)
+    assert (got == 0).all(axis=1).any()
+    # max_knn = 1: neighbourhood of one point once >= 2 are found -> zero covariance
+    g1 = pgeof.compute_features_selected(xyz, 0.2, 1, [F.Linearity, F.Surface, F.Normal_z])
+    r1 = O.compute_features_selected(xyz, 0.2, 1, [F.Linearity, F.Surface, F.Normal_z])
+    np.testing.assert_allclose(g1, r1, rtol=1e-5, atol=1e-6)
+
+
+def test_selected_type_errors():
+    with pytest.raises(TypeError):
+        pgeof.compute_features_selected(np.zeros((5, 3), np.int32), 0.1, 4, [F.Linearity])
+    with pytest.raises(TypeError):
+        pgeof.compute_features_selected(np.zeros((5, 2), np.float32), 0.1, 4, [F.Linearity])
