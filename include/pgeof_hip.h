@@ -148,13 +148,47 @@ int pgeof_compute_features_optimal_device(const float* xyz, int64_t n_xyz, const
                                           void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Serving loop (no counterpart in the reference; the reference processes one  */
+/* cloud per call on the CPU).  Requests are point clouds of any size; each    */
+/* runs pgeof_knn_features_device with k = knn on its own stream; up to        */
+/* n_slots run concurrently, admitted FIFO while the sum of their HBM          */
+/* footprints (pgeof_server_footprint) stays within hbm_budget_bytes.          */
+/* ------------------------------------------------------------------------ */
+typedef struct pgeof_server pgeof_server;
+
+/* Create on the current device; NULL on failure. */
+pgeof_server* pgeof_server_create(int n_slots, size_t hbm_budget_bytes, uint32_t knn);
+
+/* Enqueue a request.  xyz (n,3) f32 host; feat_out (n,11) f32 host;
+ * nn_out (n*knn) u32 host or NULL.  The buffers must stay valid until the
+ * request is reported by pgeof_server_wait.  PGEOF_ENOMEM if the request
+ * alone exceeds the budget, PGEOF_EINVAL if knn > n. */
+int pgeof_server_submit(pgeof_server* server, uint64_t id, const float* xyz, int64_t n, uint32_t* nn_out,
+                        float* feat_out);
+
+/* Wait up to timeout_ms for completed requests; writes up to max entries of
+ * (id, status, latency from submit to completion in ms); returns the count. */
+int pgeof_server_wait(pgeof_server* server, int max, uint64_t* ids, int* status, double* latency_ms,
+                      double timeout_ms);
+
+/* HBM bytes a request of n points occupies while in flight. */
+size_t pgeof_server_footprint(int64_t n, uint32_t knn);
+
+/* Stop the workers (requests still queued are dropped) and free the server. */
+void pgeof_server_destroy(pgeof_server* server);
+
+/* ------------------------------------------------------------------------ */
 /* Tuning and profiling (no counterpart in the reference).                   */
 /* ------------------------------------------------------------------------ */
 
 /* Set a tuning knob: "knn_occupancy" (target points per grid cell, as a
  * fraction of k), "grid_cells_per_point" (cap on grid cells / points),
  * "lds_budget" (bytes of LDS per workgroup for the search heaps),
- * "radius_cell_factor" (grid cell edge / search radius). */
+ * "radius_cell_factor" (grid cell edge / search radius), "grid_levels"
+ * (depth of the kNN grid pyramid, 1..4), "search_queue" (1: candidates pass
+ * through a small per-thread queue before the heap insert, 0: direct insert),
+ * "near_first" (1: cell rows are visited nearest-first, 0: in index order).
+ * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);
 double pgeof_get_tuning(const char* key);
 

@@ -57,6 +57,11 @@ _SIGS = {
     "pgeof_profile_enable": (None, [ctypes.c_int]),
     "pgeof_profile_report": (ctypes.c_int, [ctypes.c_char_p, _SZ]),
     "pgeof_profile_reset": (None, []),
+    "pgeof_server_create": (_P, [ctypes.c_int, _SZ, _U32]),
+    "pgeof_server_submit": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _I64, _P, _P]),
+    "pgeof_server_wait": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _D]),
+    "pgeof_server_footprint": (_SZ, [_I64, _U32]),
+    "pgeof_server_destroy": (None, [_P]),
 }
 
 
@@ -180,6 +185,53 @@ def compute_features_optimal_device(xyz, nn, nn_ptr, features, k_min=1, k_step=1
     check(lib().pgeof_compute_features_optimal_device(_ptr(xyz), xyz.shape[0], _ptr(nn), nn.shape[0], _ptr(nn_ptr),
                                                       nn_ptr.shape[0], int(k_min), int(k_step), int(k_min_search),
                                                       _ptr(features), _ptr(err), _stream(stream)))
+
+
+class Server:
+    """ctypes wrapper of the serving loop (pgeof_server_*)."""
+
+    def __init__(self, n_slots: int, hbm_budget_bytes: int, k: int):
+        self._h = lib().pgeof_server_create(int(n_slots), int(hbm_budget_bytes), int(k))
+        if not self._h:
+            raise PgeofError("pgeof_server_create failed")
+        self.k = int(k)
+        self._keep = {}
+
+    def submit(self, rid: int, xyz, feat_out, nn_out=None) -> None:
+        """xyz, feat_out (and nn_out) are C-contiguous numpy arrays kept alive
+        until the request completes."""
+        self._keep[int(rid)] = (xyz, feat_out, nn_out)
+        check(lib().pgeof_server_submit(self._h, int(rid), xyz.ctypes.data, len(xyz),
+                                        None if nn_out is None else nn_out.ctypes.data, feat_out.ctypes.data))
+
+    def wait(self, max_items: int = 64, timeout_ms: float = 1000.0):
+        import numpy as np
+
+        ids = np.zeros(max_items, np.uint64)
+        st = np.zeros(max_items, np.int32)
+        lat = np.zeros(max_items, np.float64)
+        c = lib().pgeof_server_wait(self._h, max_items, ids.ctypes.data, st.ctypes.data, lat.ctypes.data,
+                                    float(timeout_ms))
+        out = []
+        for i in range(c):
+            self._keep.pop(int(ids[i]), None)
+            out.append((int(ids[i]), int(st[i]), float(lat[i])))
+        return out
+
+    @staticmethod
+    def footprint(n: int, k: int) -> int:
+        return int(lib().pgeof_server_footprint(int(n), int(k)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().pgeof_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def knn_workspace_size(n_data: int, n_query: int, k: int) -> int:

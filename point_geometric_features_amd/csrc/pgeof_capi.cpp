@@ -56,6 +56,8 @@ struct Tuning
     double lds_budget = 65536.0;        // bytes of LDS per workgroup for heaps
     double radius_cell_factor = 0.5;    // cell edge = this * radius
     double grid_levels = 4.0;           // kNN grid pyramid depth (1..4)
+    double search_queue = 0.0;          // batch heap insertions (1) or insert directly (0)
+    double near_first = 0.0;            // visit rows nearest-first (1) or in index order (0)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -277,7 +279,7 @@ HeapPlan heap_plan(uint32_t k, bool with_pos)
 {
     const Tuning t = tuning();
     const size_t per = (size_t)k * (8 + (with_pos ? 4 : 0));
-    const size_t stage = 128 * 16;
+    const size_t stage = 128 * 16 + (size_t)kQueue * 12 * 64;  // staging + insertion queue per wave
     const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
     for (int b : {256, 128, 64})
         if (per * (size_t)b + stage * (size_t)(b / 64) <= budget) return {b, per * (size_t)b + stage * (size_t)(b / 64), false};
@@ -472,6 +474,8 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     a.k = k;
     a.margin = 1e-6f;
     a.r2 = radius * radius;
+    a.use_queue = tuning().search_queue > 0.5 ? 1 : 0;
+    a.near_first = tuning().near_first > 0.5 ? 1 : 0;
     a.out_idx = csr ? (void*)w.dense : out_idx;
     a.out_d2 = out_d2;
     a.out_feat = out_feat;
@@ -753,7 +757,7 @@ int pgeof_set_tuning(const char* key, double v)
 {
     std::lock_guard<std::mutex> l(g_tune_mu);
     const std::string k = key ? key : "";
-    if (!(v > 0.0) || !std::isfinite(v)) return fail(PGEOF_EINVAL, "tuning value must be positive");
+    if (!(v >= 0.0) || !std::isfinite(v)) return fail(PGEOF_EINVAL, "tuning value must be non-negative");
     if (k == "knn_occupancy")
         g_tune.knn_occupancy = v;
     else if (k == "grid_cells_per_point")
@@ -764,6 +768,10 @@ int pgeof_set_tuning(const char* key, double v)
         g_tune.radius_cell_factor = v;
     else if (k == "grid_levels")
         g_tune.grid_levels = v;
+    else if (k == "search_queue")
+        g_tune.search_queue = v;
+    else if (k == "near_first")
+        g_tune.near_first = v;
     else
         return fail(PGEOF_EINVAL, "unknown tuning key " + k);
     return PGEOF_OK;
@@ -778,6 +786,8 @@ double pgeof_get_tuning(const char* key)
     if (k == "lds_budget") return g_tune.lds_budget;
     if (k == "radius_cell_factor") return g_tune.radius_cell_factor;
     if (k == "grid_levels") return g_tune.grid_levels;
+    if (k == "search_queue") return g_tune.search_queue;
+    if (k == "near_first") return g_tune.near_first;
     return -1.0;
 }
 

@@ -87,58 +87,70 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
         return;
     }
 
-    Moments m, best;
-    mom_zero(m);
-    mom_zero(best);
-    double ox = 0.0, oy = 0.0, oz = 0.0;
-    uint32_t s = 0;
-    float h_best = 1.0f;
-    uint32_t k_opt = k_nn;
+    // validate the row first, so the walk below has no early exit and its
+    // gathers can be issued ahead of their use
     bool bad = false;
-    for (uint32_t j = 0; j < walk; ++j)
-    {
-        const uint32_t idx = row[j];
-        if (idx >= a.n_xyz)
-        {
-            bad = true;
-            break;
-        }
-        const float px = xyz[(size_t)idx * 3 + 0], py = xyz[(size_t)idx * 3 + 1], pz = xyz[(size_t)idx * 3 + 2];
-        if (j == 0) ox = px, oy = py, oz = pz;
-        mom_add(m, (double)px - ox, (double)py - oy, (double)pz - oz);
-        const uint32_t cnt = j + 1;
-        if (MODE == kCsrMulti)
-        {
-            while (s < a.n_scales && a.scales[s] == cnt)
-            {
-                features_of(m, cnt, f);
-                float* o = out + (size_t)s * 11;
-                for (int c = 0; c < 11; ++c) o[c] = f[c];
-                ++s;
-            }
-        }
-        else if (MODE == kCsrOptimal)
-        {
-            if (cnt >= k0 && (cnt == k0 || cnt % a.k_step == 0 || cnt == k_nn))
-            {
-                double c6[6], l[3];
-                mom_cov(m, (double)cnt, c6);
-                eig_vals_only(c6, l);
-                const float hval = eigentropy32(l);
-                if (cnt == k0 || hval < h_best)
-                {
-                    h_best = hval;
-                    k_opt = cnt;
-                    best = m;
-                }
-            }
-        }
-    }
+    for (uint32_t j = 0; j < walk; ++j) bad |= row[j] >= a.n_xyz;
     if (bad)
     {
         atomicOr(a.err, kErrNnIndex);
         for (uint32_t c = 0; c < n_out; ++c) out[c] = 0.0f;
         return;
+    }
+
+    Moments m, best;
+    mom_zero(m);
+    mom_zero(best);
+    const uint32_t i0 = row[0];
+    const double ox = xyz[(size_t)i0 * 3 + 0], oy = xyz[(size_t)i0 * 3 + 1], oz = xyz[(size_t)i0 * 3 + 2];
+    uint32_t s = 0;
+    float h_best = 1.0f;
+    uint32_t k_opt = k_nn;
+    constexpr uint32_t kChunk = 4;
+    for (uint32_t j0 = 0; j0 < walk; j0 += kChunk)
+    {
+        float px[kChunk], py[kChunk], pz[kChunk];
+#pragma unroll
+        for (uint32_t u = 0; u < kChunk; ++u)
+        {
+            const uint32_t idx = (j0 + u < walk) ? row[j0 + u] : i0;
+            px[u] = xyz[(size_t)idx * 3 + 0];
+            py[u] = xyz[(size_t)idx * 3 + 1];
+            pz[u] = xyz[(size_t)idx * 3 + 2];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kChunk; ++u)
+        {
+            if (j0 + u >= walk) break;
+            mom_add(m, (double)px[u] - ox, (double)py[u] - oy, (double)pz[u] - oz);
+            const uint32_t cnt = j0 + u + 1;
+            if (MODE == kCsrMulti)
+            {
+                while (s < a.n_scales && a.scales[s] == cnt)
+                {
+                    features_of(m, cnt, f);
+                    float* o = out + (size_t)s * 11;
+                    for (int c = 0; c < 11; ++c) o[c] = f[c];
+                    ++s;
+                }
+            }
+            else if (MODE == kCsrOptimal)
+            {
+                if (cnt >= k0 && (cnt == k0 || cnt % a.k_step == 0 || cnt == k_nn))
+                {
+                    double c6[6], l[3];
+                    mom_cov(m, (double)cnt, c6);
+                    eig_vals_only(c6, l);
+                    const float hval = eigentropy32(l);
+                    if (cnt == k0 || hval < h_best)
+                    {
+                        h_best = hval;
+                        k_opt = cnt;
+                        best = m;
+                    }
+                }
+            }
+        }
     }
     if (MODE == kCsrPlain)
     {

@@ -63,7 +63,11 @@ struct SearchArgs
     uint64_t* gheap_keys;  // global-memory heap fallback: k * n_query keys
     uint32_t* gheap_pos;   // k * n_query positions
     uint32_t* defer_flag;  // budgeted pass: 1 where the query is left to the multi-level pass
+    int use_queue;         // batch heap insertions through a per-lane LDS queue
+    int near_first;        // visit cell rows nearest-first
 };
+
+constexpr int kQueue = 4;  // entries of the per-lane insertion queue
 
 // Simple linear device arena carved out of one allocation.
 struct Arena

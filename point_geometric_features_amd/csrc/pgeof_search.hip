@@ -376,9 +376,8 @@ __device__ __forceinline__ double axis_gap(double v, int c, double lo, double h,
 //     union of their rows is compact, the wave walks that union in lockstep,
 //     stages each row's point slice into LDS with coalesced loads and every
 //     lane scans its own sub-range from LDS;
-//   - per lane: otherwise each lane walks its own rows with batched loads.
+//   - per lane: otherwise each lane walks its own rows.
 constexpr int kStage = 128;  // float4 points staged per wave per chunk
-constexpr int kBatch = 4;    // candidate loads in flight per lane (per-lane path)
 
 // Wavefront reductions over the active lanes (ROCm device library, DPP based).
 extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
@@ -410,6 +409,16 @@ __device__ __forceinline__ Box cube_box(int cx, int cy, int cz, int n, const Lev
 {
     return Box{max(cx - n, 0), min(cx + n, G.g[0] - 1), max(cy - n, 0), min(cy + n, G.g[1] - 1), max(cz - n, 0),
                min(cz + n, G.g[2] - 1)};
+}
+
+// i-th element of the order c, c-1, c+1, c-2, c+2, ... when the range [lo, hi]
+// starts at c (values outside [lo, hi] are skipped by the caller)
+__device__ __forceinline__ int near_index(int i, int c, int lo, int hi)
+{
+    if (c <= lo) return lo + i;  // plain ascending order
+    c = min(c, hi);
+    const int d = (i + 1) >> 1;
+    return (i & 1) ? c - d : c + d;
 }
 
 __device__ __forceinline__ Level pick_level(const SearchArgs& a, int l)
@@ -451,6 +460,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     }
     heap.k = a.k;
     heap.count = 0;
+    // insertion queue after the heaps: kQueue keys and positions per lane
+    unsigned char* qbase = hbase + (size_t)a.k * BLOCK * (sizeof(uint64_t) + (POS ? sizeof(uint32_t) : 0));
+    uint64_t* qk = reinterpret_cast<uint64_t*>(qbase) + threadIdx.x;
+    uint32_t* qp = reinterpret_cast<uint32_t*>(qbase + (size_t)kQueue * BLOCK * sizeof(uint64_t)) + threadIdx.x;
+    const bool use_queue = a.use_queue != 0 && !GLOBAL_HEAP;
+    uint32_t qn = 0;
 
     const float4 q4 = active ? a.queries[t] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t qorig = __float_as_uint(q4.w);
@@ -503,37 +518,56 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     auto consider = [&](const float4& p, uint32_t j) {
         const float d2 = sqdist32(qxf, qyf, qzf, p.x, p.y, p.z);
         const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
-        if (key < worst) heap.add(key, j, worst);
-    };
-
-    // per-lane scan of sorted positions [s, e) straight from global memory,
-    // kBatch loads in flight
-    auto scan_direct = [&](uint32_t s, uint32_t e) {
-        for (uint32_t j = s; j < e; j += kBatch)
+        if (key < worst)
         {
-            float4 p[kBatch];
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u)
-                if (j + u < e) p[u] = pts[j + u];
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u)
-                if (j + u < e) consider(p[u], j + u);
+            if (use_queue)
+            {
+                qk[qn * BLOCK] = key;
+                qp[qn * BLOCK] = j;
+                ++qn;
+            }
+            else
+                heap.add(key, j, worst);
         }
     };
 
-    // this lane's part [s, e) of the staged chunk [c, c + cnt), kBatch LDS
-    // reads in flight
-    auto scan_staged = [&](uint32_t s, uint32_t e, uint32_t c, uint32_t cnt) {
-        const uint32_t lo = max(s, c), hi = min(e, c + cnt);
-        for (uint32_t j = lo; j < hi; j += kBatch)
+    // move the queued candidates into the heap
+    auto flush = [&]() {
+        for (uint32_t i = 0; i < qn; ++i)
         {
-            float4 p[kBatch];
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u)
-                if (j + u < hi) p[u] = stage[j + u - c];
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u)
-                if (j + u < hi) consider(p[u], j + u);
+            const uint64_t key = qk[i * BLOCK];
+            if (key < worst) heap.add(key, qp[i * BLOCK], worst);
+        }
+        qn = 0;
+    };
+    // all active lanes flush together as soon as one queue is full
+    auto maybe_flush = [&]() {
+        if (use_queue && __ballot(qn == (uint32_t)kQueue) != 0) flush();
+    };
+
+    // per-lane scan of sorted positions [s0, e0) and [s1, e1) from global memory
+    auto scan_direct = [&](uint32_t s0, uint32_t e0, uint32_t s1, uint32_t e1) {
+        for (int pc = 0; pc < 2; ++pc)
+        {
+            const uint32_t lo = pc ? s1 : s0, hi = pc ? e1 : e0;
+            for (uint32_t j = lo; j < hi; ++j)
+            {
+                consider(pts[j], j);
+                maybe_flush();
+            }
+        }
+    };
+
+    // this lane's parts [s0, e0) and [s1, e1) of the staged chunk [c, c + cnt)
+    auto scan_staged = [&](uint32_t s0, uint32_t e0, uint32_t s1, uint32_t e1, uint32_t c, uint32_t cnt) {
+        for (int pc = 0; pc < 2; ++pc)
+        {
+            const uint32_t lo = max(pc ? s1 : s0, c), hi = min(pc ? e1 : e0, c + cnt);
+            for (uint32_t j = lo; j < hi; ++j)
+            {
+                consider(stage[j - c], j);
+                maybe_flush();
+            }
         }
     };
 
@@ -584,8 +618,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
             // every lane stages from the working level's array, also lanes
             // that sit on another level and have no row in this pass
             const float4* __restrict__ wpts = pick_level(a, lmin).pts;
-            for (int z = Z0; z <= Z1; ++z)
+            // rows nearest to the wave's centre first: close candidates fill
+            // the heaps early and most later ones fail the threshold test
+            const int zc = a.near_first ? (Z0 + Z1) / 2 : Z0, yc = a.near_first ? (Y0 + Y1) / 2 : Y0;
+            const int nz = Z1 - Z0 + 1, ny = Y1 - Y0 + 1;
+            for (int iz = 0; iz < 2 * nz; ++iz)
             {
+                const int z = near_index(iz, zc, Z0, Z1);
+                if (z < Z0 || z > Z1) continue;
                 const bool zin = has && z >= B.z0 && z <= B.z1;
                 double dz2 = 0.0;
                 if (tighten && zin)
@@ -593,8 +633,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                     const double dz = axis_gap(qz, z, G.lo[2], h, margin);
                     dz2 = dz * dz;
                 }
-                for (int y = Y0; y <= Y1; ++y)
+                for (int iy = 0; iy < 2 * ny; ++iy)
                 {
+                    const int y = near_index(iy, yc, Y0, Y1);
+                    if (y < Y0 || y > Y1) continue;
                     int a0 = 1, b0 = 0, a1 = 1, b1 = 0;
                     if (zin && y >= B.y0 && y <= B.y1) row_pieces(B, C, hasC, tighten, dz2, y, z, a0, b0, a1, b1);
                     const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
@@ -621,8 +663,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                             if (i < cnt) stage[i] = wpts[c + i];
                         }
                         wave_sync();
-                        scan_staged(s0, e0, c, cnt);
-                        scan_staged(s1, e1, c, cnt);
+                        scan_staged(s0, e0, s1, e1, c, cnt);
                         wave_sync();
                     }
                 }
@@ -630,8 +671,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         }
         else if (has)
         {
-            for (int z = B.z0; z <= B.z1; ++z)
+            const int zc = a.near_first ? cz : B.z0, yc = a.near_first ? cy : B.y0;
+            for (int iz = 0; iz < 2 * (B.z1 - B.z0 + 1); ++iz)
             {
+                const int z = near_index(iz, zc, B.z0, B.z1);
+                if (z < B.z0 || z > B.z1) continue;
                 double dz2 = 0.0;
                 if (tighten)
                 {
@@ -639,8 +683,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                     dz2 = dz * dz;
                     if (dz2 > wlim()) continue;
                 }
-                for (int y = B.y0; y <= B.y1; ++y)
+                for (int iy = 0; iy < 2 * (B.y1 - B.y0 + 1); ++iy)
                 {
+                    const int y = near_index(iy, yc, B.y0, B.y1);
+                    if (y < B.y0 || y > B.y1) continue;
                     int a0, b0, a1, b1;
                     row_pieces(B, C, hasC, tighten, dz2, y, z, a0, b0, a1, b1);
                     const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
@@ -655,11 +701,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                         s1 = cs[base + (uint32_t)a1];
                         e1 = cs[base + (uint32_t)b1 + 1];
                     }
-                    scan_direct(s0, e0);
-                    scan_direct(s1, e1);
+                    scan_direct(s0, e0, s1, e1);
                 }
             }
         }
+        if (use_queue) flush();
     };
 
     // Phase 1 (kNN): cubic shells around the query cell until the heap holds

@@ -7,6 +7,7 @@ compare indices with scipy's KD-tree, reproduced here with fixed seeds.
 """
 import numpy as np
 import pytest
+import torch
 from scipy.spatial import cKDTree
 
 import point_geometric_features_amd as pgeof
@@ -166,3 +167,35 @@ def test_knn_level_count_does_not_change_results(levels):
     oi, od = O.knn(xyz, xyz, 20, kdtree=True)
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(gd, od)
+
+
+@pytest.mark.parametrize("queue,near", [(1, 0), (0, 1), (1, 1)])
+def test_knn_scan_order_and_queue_do_not_change_results(queue, near):
+    from point_geometric_features_amd import capi
+
+    xyz = scenes.lidar_scene(60_000, seed=23, extent=25.0)
+    keys = ("search_queue", "near_first")
+    old = [capi.get_tuning(k) for k in keys]
+    try:
+        capi.set_tuning("search_queue", queue)
+        capi.set_tuning("near_first", near)
+        gi, gd = pgeof.knn_search(xyz, xyz, 20)
+        ri, rd = pgeof.radius_search(xyz, xyz, 0.5, 30)
+        x = torch.from_numpy(xyz).cuda()
+        nn_d = torch.empty(len(xyz) * 20, dtype=torch.int32, device="cuda")
+        ptr_d = torch.empty(len(xyz) + 1, dtype=torch.int32, device="cuda")
+        f_d = torch.empty((len(xyz), 11), dtype=torch.float32, device="cuda")
+        capi.knn_features_device(x, 20, nn_d, ptr_d, f_d)
+        torch.cuda.synchronize()
+        fused = f_d.cpu().numpy()
+    finally:
+        for k, v in zip(keys, old):
+            capi.set_tuning(k, v)
+    oi, od = O.knn(xyz, xyz, 20, kdtree=True)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    oi, od = O.radius(xyz, xyz, 0.5, 30, kdtree=True)
+    np.testing.assert_array_equal(ri, oi)
+    np.testing.assert_array_equal(rd, od)
+    nn, ptr = scenes.knn_csr(gi)
+    np.testing.assert_array_equal(fused, pgeof.compute_features(xyz, nn, ptr, 20))

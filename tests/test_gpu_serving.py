@@ -1,0 +1,66 @@
+"""Serving loop: every request's result is the same alone and in the mix
+(concurrent slots, uneven sizes), and equals the direct API."""
+import numpy as np
+import pytest
+
+import point_geometric_features_amd as pgeof
+from point_geometric_features_amd import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _drain(srv, n):
+    got = {}
+    while len(got) < n:
+        for rid, st, ms in srv.wait(64, timeout_ms=60_000):
+            assert st == 0
+            assert ms >= 0
+            got[rid] = ms
+    return got
+
+
+def test_serving_alone_equals_in_the_mix():
+    rng = np.random.default_rng(61)
+    sizes = [5_000, 180_000, 20_000, 90_000, 7_500, 250_000, 40_000, 12_000]
+    clouds = [scenes.lidar_scene(s, seed=100 + i, extent=200.0 * np.sqrt(s / 1e7) + 5) for i, s in enumerate(sizes)]
+    k = 20
+    alone = []
+    with capi.Server(1, 8 << 30, k) as srv:
+        for i, c in enumerate(clouds):
+            f = np.empty((len(c), 11), np.float32)
+            nn = np.empty(len(c) * k, np.uint32)
+            srv.submit(i, c, f, nn)
+            _drain(srv, 1)
+            alone.append((f, nn))
+    mixed = {}
+    with capi.Server(4, 8 << 30, k) as srv:
+        order = rng.permutation(len(clouds))
+        for i in order:
+            c = clouds[i]
+            mixed[i] = (np.empty((len(c), 11), np.float32), np.empty(len(c) * k, np.uint32))
+            srv.submit(int(i), c, mixed[i][0], mixed[i][1])
+        _drain(srv, len(clouds))
+    for i, c in enumerate(clouds):
+        np.testing.assert_array_equal(alone[i][0], mixed[i][0])
+        np.testing.assert_array_equal(alone[i][1], mixed[i][1])
+        # and the plain API agrees
+        idx, _ = pgeof.knn_search(c, c, k)
+        np.testing.assert_array_equal(idx.ravel(), mixed[i][1])
+        nn, ptr = scenes.knn_csr(idx)
+        np.testing.assert_array_equal(pgeof.compute_features(c, nn, ptr), mixed[i][0])
+
+
+def test_serving_admission_limits():
+    k = 20
+    small = scenes.uniform_cube(3000, seed=62)
+    fp = capi.Server.footprint(len(small), k)
+    with capi.Server(2, fp + 1024, k) as srv:
+        big = scenes.uniform_cube(50_000, seed=63)
+        with pytest.raises(Exception):
+            srv.submit(0, big, np.empty((len(big), 11), np.float32))  # larger than the whole budget
+        # two requests that fit only one at a time still both complete
+        outs = [np.empty((len(small), 11), np.float32) for _ in range(2)]
+        srv.submit(1, small, outs[0])
+        srv.submit(2, small, outs[1])
+        _drain(srv, 2)
+        np.testing.assert_array_equal(outs[0], outs[1])
