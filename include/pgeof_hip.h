@@ -185,9 +185,8 @@ void pgeof_server_destroy(pgeof_server* server);
  * fraction of k), "grid_cells_per_point" (cap on grid cells / points),
  * "lds_budget" (bytes of LDS per workgroup for the search heaps),
  * "radius_cell_factor" (grid cell edge / search radius), "grid_levels"
- * (depth of the kNN grid pyramid, 1..4), "search_queue" (1: candidates pass
- * through a small per-thread queue before the heap insert, 0: direct insert),
- * "near_first" (1: cell rows are visited nearest-first, 0: in index order).
+ * (depth of the kNN grid pyramid, 1..4), "near_first" (1: cell rows are
+ * visited nearest-first, 0: in index order).
  * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);
 double pgeof_get_tuning(const char* key);

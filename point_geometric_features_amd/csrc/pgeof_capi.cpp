@@ -55,9 +55,8 @@ struct Tuning
     double grid_cells_per_point = 4.0;  // cap on G / n
     double lds_budget = 65536.0;        // bytes of LDS per workgroup for heaps
     double radius_cell_factor = 0.5;    // cell edge = this * radius
-    double grid_levels = 4.0;           // kNN grid pyramid depth (1..4)
-    double search_queue = 0.0;          // batch heap insertions (1) or insert directly (0)
-    double near_first = 0.0;            // visit rows nearest-first (1) or in index order (0)
+    double grid_levels = 1.0;           // kNN grid pyramid depth (1..4)
+    double near_first = 1.0;            // visit rows nearest-first (1) or in index order (0)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -252,6 +251,7 @@ struct SearchWs
     uint32_t* foff;    // their exclusive scan
     uint32_t* fpart;
     float4* qlist;     // compacted deferred queries
+    Level* levels;     // device copy of the level table
 };
 
 constexpr uint32_t kBboxBlocks = 1024;
@@ -279,7 +279,7 @@ HeapPlan heap_plan(uint32_t k, bool with_pos)
 {
     const Tuning t = tuning();
     const size_t per = (size_t)k * (8 + (with_pos ? 4 : 0));
-    const size_t stage = 128 * 16 + (size_t)kQueue * 12 * 64;  // staging + insertion queue per wave
+    const size_t stage = 128 * 16;
     const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
     for (int b : {256, 128, 64})
         if (per * (size_t)b + stage * (size_t)(b / 64) <= budget) return {b, per * (size_t)b + stage * (size_t)(b / 64), false};
@@ -295,6 +295,7 @@ size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool with_pos, bool c
     b += 2 * al(n * 4) + kMaxLevels * al(n * 16);
     b += 2 * al(m * 4) + al(m * 16);
     b += 2 * al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4) + al(m * 16);
+    b += al(kMaxLevels * sizeof(Level));
     const HeapPlan hp = heap_plan(k, with_pos);
     if (hp.global) b += al(m * k * 8) + al(m * k * 4);
     if (csr) b += al(m * k * 4) + al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4);
@@ -321,6 +322,7 @@ SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool with_pos, boo
     w.foff = ar.take<uint32_t>(m + 1);
     w.fpart = ar.take<uint32_t>(scan_partials_needed(m + 1));
     w.qlist = ar.take<float4>(m);
+    w.levels = ar.take<Level>(kMaxLevels);
     const HeapPlan hp = heap_plan(k, with_pos);
     if (hp.global)
     {
@@ -452,6 +454,8 @@ int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, ui
         HIP_TRY(launch_scatter(query, m, w.qcell, w.qrank, w.count, w.qs, s));
         *qs_out = w.qs;
     }
+    sa.levels = w.levels;
+    HIP_TRY(launch_store_levels(sa, w.levels, s));
     return PGEOF_OK;
 }
 
@@ -474,7 +478,6 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     a.k = k;
     a.margin = 1e-6f;
     a.r2 = radius * radius;
-    a.use_queue = tuning().search_queue > 0.5 ? 1 : 0;
     a.near_first = tuning().near_first > 0.5 ? 1 : 0;
     a.out_idx = csr ? (void*)w.dense : out_idx;
     a.out_d2 = out_d2;
@@ -768,8 +771,6 @@ int pgeof_set_tuning(const char* key, double v)
         g_tune.radius_cell_factor = v;
     else if (k == "grid_levels")
         g_tune.grid_levels = v;
-    else if (k == "search_queue")
-        g_tune.search_queue = v;
     else if (k == "near_first")
         g_tune.near_first = v;
     else
@@ -786,7 +787,6 @@ double pgeof_get_tuning(const char* key)
     if (k == "lds_budget") return g_tune.lds_budget;
     if (k == "radius_cell_factor") return g_tune.radius_cell_factor;
     if (k == "grid_levels") return g_tune.grid_levels;
-    if (k == "search_queue") return g_tune.search_queue;
     if (k == "near_first") return g_tune.near_first;
     return -1.0;
 }

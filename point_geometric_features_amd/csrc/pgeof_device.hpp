@@ -175,14 +175,15 @@ __host__ __device__ __forceinline__ void eigvec_cross(const double a[6], double 
 __host__ __device__ __forceinline__ void eig_complete(const double a[6], const double u[3], double big[3], double small[3],
                                                       double& lbig, double& lsmall)
 {
-    double e[3];
+    // u x e_x, u x e_y or u x e_z for the smallest |u_i| (selects, not
+    // branches, so that e stays in registers)
     const double ux = fabs(u[0]), uy = fabs(u[1]), uz = fabs(u[2]);
-    if (ux <= uy && ux <= uz)
-        e[0] = 0.0, e[1] = u[2], e[2] = -u[1];  // u x e_x
-    else if (uy <= uz)
-        e[0] = -u[2], e[1] = 0.0, e[2] = u[0];  // u x e_y
-    else
-        e[0] = u[1], e[1] = -u[0], e[2] = 0.0;  // u x e_z
+    const bool bx = ux <= uy && ux <= uz;
+    const bool by = !bx && uy <= uz;
+    double e[3];
+    e[0] = bx ? 0.0 : (by ? -u[2] : u[1]);
+    e[1] = bx ? u[2] : (by ? 0.0 : -u[0]);
+    e[2] = bx ? -u[1] : (by ? u[0] : 0.0);
     const double ie = 1.0 / sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
     e[0] *= ie, e[1] *= ie, e[2] *= ie;
     const double f[3] = {u[1] * e[2] - u[2] * e[1], u[2] * e[0] - u[0] * e[2], u[0] * e[1] - u[1] * e[0]};
@@ -233,57 +234,58 @@ struct Eig
 // neighbourhoods where lambda_min must come out as 0.
 __host__ __device__ __forceinline__ Eig eig_full(const double c[6])
 {
-    Eig E;
     double a[6];
     const EigVals ev = eig_values(c, a);
-    if (ev.iso)
+    // The separated eigenvalue is the largest (hi) or the smallest one. The
+    // code below is branch-free (selects only) so that every array stays in
+    // registers.
+    const bool hi = ev.l2 - ev.l1 >= ev.l1 - ev.l0;
+    double u[3], big[3], small[3], lb, ls;
+    eigvec_cross(a, hi ? ev.l2 : ev.l0, u);
+    eig_complete(a, u, big, small, lb, ls);
+    // Rayleigh quotient u'Au of the separated eigenvector
+    const double au0 = a[0] * u[0] + a[1] * u[1] + a[2] * u[2];
+    const double au1 = a[1] * u[0] + a[3] * u[1] + a[4] * u[2];
+    const double au2 = a[2] * u[0] + a[4] * u[1] + a[5] * u[2];
+    const double lu = u[0] * au0 + u[1] * au1 + u[2] * au2;
+    // hi: (l0, l1, l2) = (small, big, u); otherwise (u, small, big)
+    double l[3] = {hi ? ls : lu, hi ? lb : ls, hi ? lu : lb};
+    double v[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
     {
-        // Eigen returns the identity basis for a multiple of the identity
-        E.l[0] = ev.l0 * ev.scale;
-        E.l[1] = ev.l1 * ev.scale;
-        E.l[2] = ev.l2 * ev.scale;
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) E.v[i][j] = (i == j) ? 1.0 : 0.0;
-        return E;
-    }
-    double l[3];
-    const double gap_hi = ev.l2 - ev.l1, gap_lo = ev.l1 - ev.l0;
-    int sep;
-    if (gap_hi >= gap_lo)
-    {
-        sep = 2;
-        eigvec_cross(a, ev.l2, E.v[2]);
-        eig_complete(a, E.v[2], E.v[1], E.v[0], l[1], l[0]);
-    }
-    else
-    {
-        sep = 0;
-        eigvec_cross(a, ev.l0, E.v[0]);
-        eig_complete(a, E.v[0], E.v[2], E.v[1], l[2], l[1]);
-    }
-    {
-        const double* u = E.v[sep];
-        const double au0 = a[0] * u[0] + a[1] * u[1] + a[2] * u[2];
-        const double au1 = a[1] * u[0] + a[3] * u[1] + a[4] * u[2];
-        const double au2 = a[2] * u[0] + a[4] * u[1] + a[5] * u[2];
-        l[sep] = u[0] * au0 + u[1] * au1 + u[2] * au2;
+        v[0][j] = hi ? small[j] : u[j];
+        v[1][j] = hi ? big[j] : small[j];
+        v[2][j] = hi ? u[j] : big[j];
     }
     // ascending order (a swap can only happen within rounding)
+#pragma unroll
     for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
         for (int i = 0; i < 2; ++i)
-            if (l[i] > l[i + 1])
+        {
+            const bool sw = l[i] > l[i + 1];
+            const double tl = l[i];
+            l[i] = sw ? l[i + 1] : l[i];
+            l[i + 1] = sw ? tl : l[i + 1];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
             {
-                const double tl = l[i];
-                l[i] = l[i + 1];
-                l[i + 1] = tl;
-                for (int j = 0; j < 3; ++j)
-                {
-                    const double tv = E.v[i][j];
-                    E.v[i][j] = E.v[i + 1][j];
-                    E.v[i + 1][j] = tv;
-                }
+                const double tv = v[i][j];
+                v[i][j] = sw ? v[i + 1][j] : v[i][j];
+                v[i + 1][j] = sw ? tv : v[i + 1][j];
             }
-    for (int i = 0; i < 3; ++i) E.l[i] = l[i] * ev.scale;
+        }
+    // Eigen returns the identity basis for a multiple of the identity
+    const double li[3] = {ev.l0, ev.l1, ev.l2};
+    Eig E;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        E.l[i] = (ev.iso ? li[i] : l[i]) * ev.scale;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) E.v[i][j] = ev.iso ? (i == j ? 1.0 : 0.0) : v[i][j];
+    }
     return E;
 }
 

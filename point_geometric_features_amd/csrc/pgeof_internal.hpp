@@ -50,6 +50,7 @@ constexpr int kMaxLevels = 4;
 struct SearchArgs
 {
     Level lv[kMaxLevels];  // level l has cell edge h_0 * 4^l
+    const Level* levels;   // the same table in device memory (per-lane level choice reads it)
     int n_levels;
     const float4* queries;  // sorted queries (x, y, z, bits of original query index)
     uint32_t n_query;
@@ -63,11 +64,8 @@ struct SearchArgs
     uint64_t* gheap_keys;  // global-memory heap fallback: k * n_query keys
     uint32_t* gheap_pos;   // k * n_query positions
     uint32_t* defer_flag;  // budgeted pass: 1 where the query is left to the multi-level pass
-    int use_queue;         // batch heap insertions through a per-lane LDS queue
     int near_first;        // visit cell rows nearest-first
 };
-
-constexpr int kQueue = 4;  // entries of the per-lane insertion queue
 
 // Simple linear device arena carved out of one allocation.
 struct Arena
@@ -98,6 +96,7 @@ hipError_t launch_scatter(const float* xyz, uint32_t n, const uint32_t* cell, co
                           const uint32_t* cell_start, float4* pts, hipStream_t s);
 hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds_bytes, bool global_heap, bool budget,
                          hipStream_t s);
+hipError_t launch_store_levels(const SearchArgs& a, Level* out, hipStream_t s);
 hipError_t launch_compact_queries(const float4* q, uint32_t n, const uint32_t* flag, const uint32_t* off,
                                   float4* out, hipStream_t s);
 hipError_t launch_arange_ptr(uint32_t* nn_ptr, uint32_t n_points, uint32_t k, hipStream_t s);

@@ -421,14 +421,10 @@ __device__ __forceinline__ int near_index(int i, int c, int lo, int hi)
     return (i & 1) ? c - d : c + d;
 }
 
-__device__ __forceinline__ Level pick_level(const SearchArgs& a, int l)
-{
-    Level G = a.lv[0];
-    if (l == 1) G = a.lv[1];
-    if (l == 2) G = a.lv[2];
-    if (l == 3) G = a.lv[3];
-    return G;
-}
+// Level l of the pyramid.  A level that differs between lanes is read from
+// the device copy of the table: indexing the kernel-argument copy with a
+// per-lane value would move the whole argument block to scratch memory.
+__device__ __forceinline__ Level pick_level(const SearchArgs& a, int l) { return a.levels[l]; }
 
 template <int MODE, int WAVES, bool GLOBAL_HEAP, bool BUDGET>
 __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
@@ -460,12 +456,6 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     }
     heap.k = a.k;
     heap.count = 0;
-    // insertion queue after the heaps: kQueue keys and positions per lane
-    unsigned char* qbase = hbase + (size_t)a.k * BLOCK * (sizeof(uint64_t) + (POS ? sizeof(uint32_t) : 0));
-    uint64_t* qk = reinterpret_cast<uint64_t*>(qbase) + threadIdx.x;
-    uint32_t* qp = reinterpret_cast<uint32_t*>(qbase + (size_t)kQueue * BLOCK * sizeof(uint64_t)) + threadIdx.x;
-    const bool use_queue = a.use_queue != 0 && !GLOBAL_HEAP;
-    uint32_t qn = 0;
 
     const float4 q4 = active ? a.queries[t] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t qorig = __float_as_uint(q4.w);
@@ -498,7 +488,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
             if (count >= a.k) break;
         }
     }
-    const Level G = pick_level(a, level);
+    const Level G = (RADIUS || BUDGET) ? a.lv[0] : pick_level(a, level);
     const float4* __restrict__ pts = G.pts;
     const uint32_t* __restrict__ cs = G.cs;
     const int G0 = G.g[0], G1 = G.g[1], G2 = G.g[2];
@@ -518,43 +508,16 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     auto consider = [&](const float4& p, uint32_t j) {
         const float d2 = sqdist32(qxf, qyf, qzf, p.x, p.y, p.z);
         const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
-        if (key < worst)
-        {
-            if (use_queue)
-            {
-                qk[qn * BLOCK] = key;
-                qp[qn * BLOCK] = j;
-                ++qn;
-            }
-            else
-                heap.add(key, j, worst);
-        }
+        if (key < worst) heap.add(key, j, worst);
     };
 
-    // move the queued candidates into the heap
-    auto flush = [&]() {
-        for (uint32_t i = 0; i < qn; ++i)
-        {
-            const uint64_t key = qk[i * BLOCK];
-            if (key < worst) heap.add(key, qp[i * BLOCK], worst);
-        }
-        qn = 0;
-    };
-    // all active lanes flush together as soon as one queue is full
-    auto maybe_flush = [&]() {
-        if (use_queue && __ballot(qn == (uint32_t)kQueue) != 0) flush();
-    };
 
     // per-lane scan of sorted positions [s0, e0) and [s1, e1) from global memory
     auto scan_direct = [&](uint32_t s0, uint32_t e0, uint32_t s1, uint32_t e1) {
         for (int pc = 0; pc < 2; ++pc)
         {
             const uint32_t lo = pc ? s1 : s0, hi = pc ? e1 : e0;
-            for (uint32_t j = lo; j < hi; ++j)
-            {
-                consider(pts[j], j);
-                maybe_flush();
-            }
+            for (uint32_t j = lo; j < hi; ++j) consider(pts[j], j);
         }
     };
 
@@ -563,40 +526,35 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         for (int pc = 0; pc < 2; ++pc)
         {
             const uint32_t lo = max(pc ? s1 : s0, c), hi = min(pc ? e1 : e0, c + cnt);
-            for (uint32_t j = lo; j < hi; ++j)
-            {
-                consider(stage[j - c], j);
-                maybe_flush();
-            }
+            for (uint32_t j = lo; j < hi; ++j) consider(stage[j - c], j);
         }
     };
 
-    // this lane's x pieces of row (y, z) of box B minus box C
-    auto row_pieces = [&](const Box& B, const Box& C, bool hasC, bool tighten, double dz2, int y, int z, int& a0,
-                          int& b0, int& a1, int& b1) {
-        a0 = 1, b0 = 0, a1 = 1, b1 = 0;
+    // this lane's x pieces [p.x, p.y] and [p.z, p.w] of row (y, z) of box B
+    // minus box C (a piece is empty when its start exceeds its end); returned
+    // by value so that the pieces stay in registers
+    auto row_pieces = [&](const Box& B, const Box& C, bool hasC, bool tighten, double dz2, int y, int z) -> int4 {
         int xa = B.x0, xb = B.x1;
         if (tighten)
         {
             const double wl = wlim();
             const double dy = axis_gap(qy, y, G.lo[1], h, margin);
             const double rem = wl - dz2 - dy * dy;
-            if (rem < 0.0) return;
-            if (rem < INFINITY)
+            if (rem < 0.0) xb = xa - 1;
+            else if (rem < INFINITY)
             {
                 const double xr = sqrt(rem) + margin;
                 xa = max(xa, cell_coord(qx - xr, G.lo[0], G.inv_h, G0));
                 xb = min(xb, cell_coord(qx + xr, G.lo[0], G.inv_h, G0));
             }
         }
-        if (xa > xb) return;
-        if (hasC && y >= C.y0 && y <= C.y1 && z >= C.z0 && z <= C.z1)
-        {
-            a0 = xa, b0 = min(xb, C.x0 - 1);
-            a1 = max(xa, C.x1 + 1), b1 = xb;
-        }
-        else
-            a0 = xa, b0 = xb;
+        const bool split = hasC && y >= C.y0 && y <= C.y1 && z >= C.z0 && z <= C.z1;
+        int4 p;
+        p.x = xa;
+        p.y = split ? min(xb, C.x0 - 1) : xb;
+        p.z = split ? max(xa, C.x1 + 1) : 1;
+        p.w = split ? xb : 0;
+        return p;
     };
 
     // Scan the cells of box B minus box C (per lane; empty boxes allowed).
@@ -617,7 +575,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         {
             // every lane stages from the working level's array, also lanes
             // that sit on another level and have no row in this pass
-            const float4* __restrict__ wpts = pick_level(a, lmin).pts;
+            const float4* __restrict__ wpts = (RADIUS || BUDGET) ? a.lv[0].pts : pick_level(a, lmin).pts;
             // rows nearest to the wave's centre first: close candidates fill
             // the heaps early and most later ones fail the threshold test
             const int zc = a.near_first ? (Z0 + Z1) / 2 : Z0, yc = a.near_first ? (Y0 + Y1) / 2 : Y0;
@@ -637,8 +595,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                 {
                     const int y = near_index(iy, yc, Y0, Y1);
                     if (y < Y0 || y > Y1) continue;
-                    int a0 = 1, b0 = 0, a1 = 1, b1 = 0;
-                    if (zin && y >= B.y0 && y <= B.y1) row_pieces(B, C, hasC, tighten, dz2, y, z, a0, b0, a1, b1);
+                    int4 rp = make_int4(1, 0, 1, 0);
+                    if (zin && y >= B.y0 && y <= B.y1) rp = row_pieces(B, C, hasC, tighten, dz2, y, z);
+                    const int a0 = rp.x, b0 = rp.y, a1 = rp.z, b1 = rp.w;
                     const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
                     uint32_t s0 = 0xFFFFFFFFu, e0 = 0, s1 = 0xFFFFFFFFu, e1 = 0;
                     if (a0 <= b0)
@@ -687,8 +646,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                 {
                     const int y = near_index(iy, yc, B.y0, B.y1);
                     if (y < B.y0 || y > B.y1) continue;
-                    int a0, b0, a1, b1;
-                    row_pieces(B, C, hasC, tighten, dz2, y, z, a0, b0, a1, b1);
+                    const int4 rp = row_pieces(B, C, hasC, tighten, dz2, y, z);
+                    const int a0 = rp.x, b0 = rp.y, a1 = rp.z, b1 = rp.w;
                     const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
                     uint32_t s0 = 0, e0 = 0, s1 = 0, e1 = 0;
                     if (a0 <= b0)
@@ -705,7 +664,6 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                 }
             }
         }
-        if (use_queue) flush();
     };
 
     // Phase 1 (kNN): cubic shells around the query cell until the heap holds
@@ -1233,6 +1191,22 @@ hipError_t launch_hist(const float* xyz, uint32_t n, const GridParams& gp, uint3
         k_hist_small<<<(n + 256 * 16 - 1) / (256 * 16), 256, 0, s>>>(xyz, n, gp, count, cell, rank);
     else
         k_hist<<<(n + 255) / 256, 256, 0, s>>>(xyz, n, gp, count, cell, rank);
+    return hipGetLastError();
+}
+
+__global__ void k_store_levels(SearchArgs a, Level* out)
+{
+    if (threadIdx.x != 0) return;
+    out[0] = a.lv[0];
+    out[1] = a.lv[1];
+    out[2] = a.lv[2];
+    out[3] = a.lv[3];
+}
+
+hipError_t launch_store_levels(const SearchArgs& a, Level* out, hipStream_t s)
+{
+    static_assert(kMaxLevels == 4, "k_store_levels copies four levels");
+    k_store_levels<<<1, 64, 0, s>>>(a, out);
     return hipGetLastError();
 }
 

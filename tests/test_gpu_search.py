@@ -169,15 +169,14 @@ def test_knn_level_count_does_not_change_results(levels):
     np.testing.assert_array_equal(gd, od)
 
 
-@pytest.mark.parametrize("queue,near", [(1, 0), (0, 1), (1, 1)])
-def test_knn_scan_order_and_queue_do_not_change_results(queue, near):
+@pytest.mark.parametrize("near", [0, 1])
+def test_knn_scan_order_does_not_change_results(near):
     from point_geometric_features_amd import capi
 
     xyz = scenes.lidar_scene(60_000, seed=23, extent=25.0)
-    keys = ("search_queue", "near_first")
+    keys = ("near_first",)
     old = [capi.get_tuning(k) for k in keys]
     try:
-        capi.set_tuning("search_queue", queue)
         capi.set_tuning("near_first", near)
         gi, gd = pgeof.knn_search(xyz, xyz, 20)
         ri, rd = pgeof.radius_search(xyz, xyz, 0.5, 30)
@@ -194,7 +193,7 @@ def test_knn_scan_order_and_queue_do_not_change_results(queue, near):
     oi, od = O.knn(xyz, xyz, 20, kdtree=True)
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(gd, od)
-    oi, od = O.radius(xyz, xyz, 0.5, 30, kdtree=True)
+    oi, od, _ = O.radius(xyz, xyz, 0.5, 30, kdtree=True)
     np.testing.assert_array_equal(ri, oi)
     np.testing.assert_array_equal(rd, od)
     nn, ptr = scenes.knn_csr(gi)
