@@ -273,12 +273,14 @@ HeapPlan heap_plan_bytes(uint32_t k, size_t entry_bytes)
     return {256, 0, true};
 }
 
-// Search kernels: per wave 64 heaps of k entries plus a 128-point staging
-// buffer (2 KiB) in LDS.
+// Search kernels: per wave 64 heaps plus a 128-point staging buffer (2 KiB)
+// in LDS.  A heap keeps its root in registers and nodes 1..k-1 in groups of
+// four slots: ceil((k - 1) / 4) groups of four 8-byte keys (and 4-byte
+// positions) per lane.
 HeapPlan heap_plan(uint32_t k, bool with_pos)
 {
     const Tuning t = tuning();
-    const size_t per = (size_t)k * (8 + (with_pos ? 4 : 0));
+    const size_t per = (size_t)4 * ((k + 2) / 4) * (8 + (with_pos ? 4 : 0));
     const size_t stage = 128 * 16;
     const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
     for (int b : {256, 128, 64})

@@ -15,6 +15,8 @@
 // ((dx*dx + dy*dy) + dz*dz) with dx = query - point and no contraction, so
 // the neighbour lists and distances are those of an exact search with that
 // metric.
+#include <type_traits>
+
 #include "pgeof_device.hpp"
 #include "pgeof_internal.hpp"
 
@@ -275,6 +277,8 @@ struct Heap
     uint32_t count;  // entries held; the heap property holds once count == k
 
     __device__ __forceinline__ uint64_t top() const { return keys[0]; }
+    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return keys[i * stride]; }
+    __device__ __forceinline__ uint32_t pos_at(uint32_t i) const { return POS ? pos[i * stride] : 0u; }
 
     // put `key` at index i of heap [0, n) and sift it down
     __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t p, uint32_t n)
@@ -353,6 +357,104 @@ struct Heap
             }
             keys[end * stride] = top_k;
             if (POS) pos[end * stride] = tp;
+            sift_from(0, last, lp, end);
+        }
+    }
+};
+
+// 4-ary max-heap of k 64-bit keys (and positions) for one lane.  Node 0, the
+// largest key, is held in registers; nodes 1..k-1 live in LDS with the four
+// children 4i+1..4i+4 of node i stored as one aligned group of four
+// consecutive slots of this lane, read with two 16-byte loads.  A sift from
+// the root of a 20-entry heap touches two groups (binary heap: four levels).
+template <bool POS>
+struct Heap4
+{
+    uint64_t* keys;    // this lane's group 0; group g at keys + g * gstride
+    uint32_t* pos;
+    uint32_t gstride;  // entries between consecutive groups of one lane (4 * block size)
+    uint32_t k;
+    uint32_t count;    // entries held; the heap property holds once count == k
+    uint64_t rkey;     // node 0
+    uint32_t rpos;
+
+    __device__ __forceinline__ uint32_t slot(uint32_t i) const { return ((i - 1) >> 2) * gstride + ((i - 1) & 3); }
+    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return i == 0 ? rkey : keys[slot(i)]; }
+    __device__ __forceinline__ uint32_t pos_at(uint32_t i) const { return i == 0 ? rpos : (POS ? pos[slot(i)] : 0u); }
+    __device__ __forceinline__ void put(uint32_t i, uint64_t kk, uint32_t p)
+    {
+        if (i == 0)
+        {
+            rkey = kk;
+            rpos = p;
+        }
+        else
+        {
+            keys[slot(i)] = kk;
+            if (POS) pos[slot(i)] = p;
+        }
+    }
+
+    // put `key` at node i of the heap of nodes [0, n) and sift it down
+    __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t p, uint32_t n)
+    {
+        for (;;)
+        {
+            const uint32_t c0 = 4 * i + 1;
+            if (c0 >= n) break;
+            const ulonglong2* g = reinterpret_cast<const ulonglong2*>(keys + i * gstride);
+            const ulonglong2 g01 = g[0], g23 = g[1];
+            uint64_t mk = g01.x;
+            uint32_t mj = 0;
+            if (c0 + 1 < n && g01.y > mk) mk = g01.y, mj = 1;
+            if (c0 + 2 < n && g23.x > mk) mk = g23.x, mj = 2;
+            if (c0 + 3 < n && g23.y > mk) mk = g23.y, mj = 3;
+            if (mk <= key) break;
+            put(i, mk, POS ? pos[i * gstride + mj] : 0u);
+            i = c0 + mj;
+        }
+        put(i, key, p);
+    }
+
+    __device__ __forceinline__ void heapify()
+    {
+        if (k < 2) return;
+        for (int i = (int)(k - 2) / 4; i >= 0; --i) sift_from((uint32_t)i, key_at(i), pos_at(i), k);
+    }
+
+    // precondition: key < worst (worst = sentinel until the heap is full)
+    __device__ __forceinline__ void add(uint64_t key, uint32_t p, uint64_t& worst)
+    {
+        if (count < k)
+        {
+            put(count, key, p);
+            if (++count == k)
+            {
+                heapify();
+                worst = rkey;
+            }
+        }
+        else
+        {
+            sift_from(0, key, p, k);
+            worst = rkey;
+        }
+    }
+
+    // pad with the sentinel, restore the heap property, sort ascending
+    __device__ __forceinline__ void finish(uint64_t sentinel)
+    {
+        if (count < k)
+        {
+            for (uint32_t j = count; j < k; ++j) put(j, sentinel, 0xFFFFFFFFu);
+            heapify();
+        }
+        for (uint32_t end = k - 1; end > 0; --end)
+        {
+            const uint64_t last = keys[slot(end)];
+            const uint32_t lp = POS ? pos[slot(end)] : 0u;
+            keys[slot(end)] = rkey;
+            if (POS) pos[slot(end)] = rpos;
             sift_from(0, last, lp, end);
         }
     }
@@ -440,8 +542,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     float4* stage = reinterpret_cast<float4*>(smem) + wid * kStage;
     unsigned char* hbase = smem + (size_t)WAVES * kStage * sizeof(float4);
 
-    Heap<POS> heap;
-    if (GLOBAL_HEAP)
+    // LDS heaps: 4-ary with the root in registers; global-memory fallback for
+    // large k: binary
+    using HeapT = typename std::conditional<GLOBAL_HEAP, Heap<POS>, Heap4<POS>>::type;
+    HeapT heap;
+    if constexpr (GLOBAL_HEAP)
     {
         // inactive lanes never touch their (non-existent) global heap
         heap.keys = a.gheap_keys + (active ? t : 0);
@@ -450,9 +555,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     }
     else
     {
-        heap.keys = reinterpret_cast<uint64_t*>(hbase) + threadIdx.x;
-        heap.pos = reinterpret_cast<uint32_t*>(hbase + (size_t)a.k * BLOCK * sizeof(uint64_t)) + threadIdx.x;
-        heap.stride = BLOCK;
+        // ceil((k - 1) / 4) groups of four keys (and four positions) per lane
+        const uint32_t groups = (a.k + 2) / 4;
+        heap.keys = reinterpret_cast<uint64_t*>(hbase) + (size_t)threadIdx.x * 4;
+        heap.pos = reinterpret_cast<uint32_t*>(hbase + (size_t)groups * BLOCK * 4 * sizeof(uint64_t)) +
+                   (size_t)threadIdx.x * 4;
+        heap.gstride = 4 * BLOCK;
+        heap.rkey = 0;
+        heap.rpos = 0;
     }
     heap.k = a.k;
     heap.count = 0;
@@ -738,7 +848,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         float* od = a.out_d2 ? a.out_d2 + (size_t)qorig * k : nullptr;
         for (uint32_t j = 0; j < k; ++j)
         {
-            const uint64_t key = heap.keys[j * heap.stride];
+            const uint64_t key = heap.key_at(j);
             oi[j] = (uint32_t)key;
             if (od) od[j] = key == ~0ull ? INFINITY : __uint_as_float((uint32_t)(key >> 32));
         }
@@ -752,7 +862,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
             float* od = a.out_d2 + (size_t)qorig * k;
             for (uint32_t j = 0; j < k; ++j)
             {
-                const uint64_t key = heap.keys[j * heap.stride];
+                const uint64_t key = heap.key_at(j);
                 if (key != sentinel)
                 {
                     oi[j] = (int32_t)(uint32_t)key;
@@ -773,7 +883,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
             uint32_t* oi = reinterpret_cast<uint32_t*>(a.out_idx) + (size_t)qorig * k;
             for (uint32_t j = 0; j < k; ++j)
             {
-                const uint64_t key = heap.keys[j * heap.stride];
+                const uint64_t key = heap.key_at(j);
                 if (key != sentinel) oi[cnt++] = (uint32_t)key;
             }
         }
@@ -791,8 +901,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
             double ox = 0.0, oy = 0.0, oz = 0.0;
             for (uint32_t j = 0; j < k; ++j)
             {
-                const uint64_t key = heap.keys[j * heap.stride];
-                const uint32_t p = heap.pos[j * heap.stride];
+                const uint64_t key = heap.key_at(j);
+                const uint32_t p = heap.pos_at(j);
                 const float4 pt = pts[p];
                 if (j == 0) ox = pt.x, oy = pt.y, oz = pt.z;
                 mom_add(m, (double)pt.x - ox, (double)pt.y - oy, (double)pt.z - oz);
