@@ -41,16 +41,17 @@ def _parse():
     return ap.parse_args()
 
 
-def _pmc_traffic(kernel: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    calibrated on a copy kernel; committed summary, or None."""
+def _pmc_traffic(kernel: str, workload: str):
+    """HBM bytes per step of `kernel` on `workload`, measured with rocprofv3
+    --pmc (FETCH_SIZE, WRITE_SIZE) and calibrated on a copy kernel of known
+    size; read from the committed summary, or None when not measured."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    e = d.get(kernel)
-    return None if e is None else float(e["bytes_per_launch"])
+    e = d.get(f"{kernel}@{workload}")
+    return None if e is None else float(e["bytes_per_step"])
 
 
 def _cpu_baseline(xyz: np.ndarray, k: int, sample: int):
@@ -197,7 +198,10 @@ def main():
         def step():
             capi.knn_features_device(x, k, nn, ptr, feat, workspace=ws, stream=stream)
 
-        kernel = "knn_features_fused"
+        # one search kernel, launched as the budgeted pass over every point
+        # and the unbudgeted pass over the deferred ones: its time per step is
+        # the sum of both launches
+        kernel = ("knn_features_fused_dense", "knn_features_fused")
         bytes_per_point = 60 + 4 * k  # xyz 12 + nn_ptr 4 + nn 4k + features 44
         units = n
         metric_unit = "Mpoints/s"
@@ -288,12 +292,15 @@ def main():
     total = t1 - t0
     ms_per_step = total / args.steps * 1e3
     value = units * args.steps / total / 1e6
-    launches, kms = prof.get(kernel, (0, 0.0))
-    avg_kernel_ms = kms / max(launches, 1)
+    names = kernel if isinstance(kernel, tuple) else (kernel,)
+    launches = sum(prof.get(kk, (0, 0.0))[0] for kk in names)
+    # kernel time per step (all launches of the kernel within one step)
+    avg_kernel_ms = sum(prof.get(kk, (0, 0.0))[1] for kk in names) / args.steps
+    kernel = "+".join(names)
     roofline = None
     if bytes_per_point is not None and launches:
         achieved = bytes_per_point * units / (avg_kernel_ms * 1e-3) / 1e9
-        traffic = _pmc_traffic(kernel)
+        traffic = _pmc_traffic(kernel, wl)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": kernel, "kernel_ms": round(avg_kernel_ms, 4),

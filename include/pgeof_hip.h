@@ -186,7 +186,10 @@ void pgeof_server_destroy(pgeof_server* server);
  * "lds_budget" (bytes of LDS per workgroup for the search heaps),
  * "radius_cell_factor" (grid cell edge / search radius), "grid_levels"
  * (depth of the kNN grid pyramid, 1..4), "near_first" (1: cell rows are
- * visited nearest-first, 0: in index order).
+ * visited nearest-first, 0: in index order), "budget_cells" (budgeted kNN
+ * pass: a query whose ball reaches farther than this many cells from its own
+ * cell is deferred to the next pass; 1..64), "grid_level_ratio" (cell edge
+ * ratio between consecutive pyramid levels, 2..8).
  * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);
 double pgeof_get_tuning(const char* key);

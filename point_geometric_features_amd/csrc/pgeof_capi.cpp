@@ -57,6 +57,8 @@ struct Tuning
     double radius_cell_factor = 0.5;    // cell edge = this * radius
     double grid_levels = 1.0;           // kNN grid pyramid depth (1..4)
     double near_first = 1.0;            // visit rows nearest-first (1) or in index order (0)
+    double budget_cells = 2.0;          // budgeted kNN pass: ball half-width limit in cells
+    double grid_level_ratio = 2.0;      // cell edge ratio between consecutive pyramid levels (2..8)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -243,7 +245,6 @@ struct SearchWs
     uint32_t* qrank;
     float4* qs;
     uint64_t* gheap_keys;
-    uint32_t* gheap_pos;
     uint32_t* dense;   // radius csr: n_query * cap
     uint32_t* counts;  // radius csr: n_query (+1 for the scan)
     uint32_t* csr_part;
@@ -251,6 +252,10 @@ struct SearchWs
     uint32_t* foff;    // their exclusive scan
     uint32_t* fpart;
     float4* qlist;     // compacted deferred queries
+    float4* qlist2;    // the same, ping-pong partner for the next level
+    float* dbound;     // per slot of a budgeted pass: bound of the k-th distance of a deferred query
+    float* blist;      // bounds of the compacted deferred queries
+    float* blist2;
     Level* levels;     // device copy of the level table
 };
 
@@ -274,13 +279,12 @@ HeapPlan heap_plan_bytes(uint32_t k, size_t entry_bytes)
 }
 
 // Search kernels: per wave 64 heaps plus a 128-point staging buffer (2 KiB)
-// in LDS.  A heap keeps its root in registers and nodes 1..k-1 in groups of
-// four slots: ceil((k - 1) / 4) groups of four 8-byte keys (and 4-byte
-// positions) per lane.
-HeapPlan heap_plan(uint32_t k, bool with_pos)
+// in LDS.  A heap keeps its root in a register and nodes 1..k-1 in groups of
+// four slots: ceil((k - 1) / 4) groups of four 8-byte keys per lane.
+HeapPlan heap_plan(uint32_t k)
 {
     const Tuning t = tuning();
-    const size_t per = (size_t)4 * ((k + 2) / 4) * (8 + (with_pos ? 4 : 0));
+    const size_t per = (size_t)4 * ((k + 2) / 4) * 8;
     const size_t stage = 128 * 16;
     const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
     for (int b : {256, 128, 64})
@@ -288,7 +292,7 @@ HeapPlan heap_plan(uint32_t k, bool with_pos)
     return {256, 0, true};
 }
 
-size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool with_pos, bool csr)
+size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool csr)
 {
     const uint64_t G = grid_cap(n) + 1;
     size_t b = 0;
@@ -296,15 +300,15 @@ size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool with_pos, bool c
     b += al(G * 4) + al((2 * G + 64) * 4) + al(scan_partials_needed(G) * 4);
     b += 2 * al(n * 4) + kMaxLevels * al(n * 16);
     b += 2 * al(m * 4) + al(m * 16);
-    b += 2 * al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4) + al(m * 16);
+    b += 2 * al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4) + 2 * al(m * 16) + 3 * al(m * 4);
     b += al(kMaxLevels * sizeof(Level));
-    const HeapPlan hp = heap_plan(k, with_pos);
-    if (hp.global) b += al(m * k * 8) + al(m * k * 4);
+    const HeapPlan hp = heap_plan(k);
+    if (hp.global) b += al(m * k * 8);
     if (csr) b += al(m * k * 4) + al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4);
     return b + 4096;
 }
 
-SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool with_pos, bool csr)
+SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool csr)
 {
     SearchWs w{};
     const uint64_t G = grid_cap(n) + 1;
@@ -324,13 +328,13 @@ SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool with_pos, boo
     w.foff = ar.take<uint32_t>(m + 1);
     w.fpart = ar.take<uint32_t>(scan_partials_needed(m + 1));
     w.qlist = ar.take<float4>(m);
+    w.qlist2 = ar.take<float4>(m);
+    w.dbound = ar.take<float>(m);
+    w.blist = ar.take<float>(m);
+    w.blist2 = ar.take<float>(m);
     w.levels = ar.take<Level>(kMaxLevels);
-    const HeapPlan hp = heap_plan(k, with_pos);
-    if (hp.global)
-    {
-        w.gheap_keys = ar.take<uint64_t>(m * k);
-        w.gheap_pos = ar.take<uint32_t>(m * k);
-    }
+    const HeapPlan hp = heap_plan(k);
+    if (hp.global) w.gheap_keys = ar.take<uint64_t>(m * k);
     if (csr)
     {
         w.dense = ar.take<uint32_t>(m * k);
@@ -413,7 +417,7 @@ int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, ui
     sa.n_levels = 0;
     for (int l = 0; l < max_levels; ++l)
     {
-        const double hl = h * std::pow(4.0, (double)l);
+        const double hl = h * std::pow(std::max(2.0, std::min(8.0, t.grid_level_ratio)), (double)l);
         const GridParams gp = make_grid(bb, hl);
         if (l > 0 && (cs_used + gp.G + 1 > cs_room || sa.lv[l - 1].g[0] * (uint64_t)sa.lv[l - 1].g[1] *
                                                               (uint64_t)sa.lv[l - 1].g[2] <= 64))
@@ -466,47 +470,72 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
                void* out_idx, float* out_d2, float* out_feat, uint32_t* out_count, Arena& ar, hipStream_t s,
                SearchWs* ws_out = nullptr)
 {
-    const bool with_pos = mode == kModeKnnFeat;
     const bool csr = mode == kModeRadiusCsr;
-    SearchWs w = carve(ar, n, m, k, with_pos, csr);
+    SearchWs w = carve(ar, n, m, k, csr);
     if (ar.off > ar.size) return fail(PGEOF_ENOMEM, "workspace too small");
     SearchArgs a{};
     const float4* qs = nullptr;
     const bool is_radius = mode == kModeRadius || mode == kModeRadiusCsr;
     int rc = build_grid(data, n, query, m, k, radius, is_radius, w, a, &qs, s);
     if (rc) return rc;
+    a.data = data;
     a.queries = qs;
     a.n_query = m;
     a.k = k;
     a.margin = 1e-6f;
     a.r2 = radius * radius;
     a.near_first = tuning().near_first > 0.5 ? 1 : 0;
+    a.budget_cells = std::max(1, std::min(64, (int)tuning().budget_cells));
     a.out_idx = csr ? (void*)w.dense : out_idx;
     a.out_d2 = out_d2;
     a.out_feat = out_feat;
     a.out_count = csr ? w.counts : out_count;
-    const HeapPlan hp = heap_plan(k, with_pos);
+    const HeapPlan hp = heap_plan(k);
     a.gheap_keys = w.gheap_keys;
-    a.gheap_pos = w.gheap_pos;
     if (mode == kModeKnn || mode == kModeKnnFeat)
     {
-        // pass A: level 0 with a bounded cell budget; queries whose k-th
-        // neighbour lies farther out are flagged and searched again, compacted
-        // and in sorted order, on the full grid pyramid (pass B)
-        a.defer_flag = w.flags;
-        HIP_TRY(launch_search(mode, a, hp.block, hp.lds, hp.global, true, s));
-        HIP_TRY(launch_fill_u32(w.flags + m, 1, 0u, s));
-        HIP_TRY(launch_scan_exclusive(w.flags, w.foff, (uint64_t)m + 1, w.fpart, s));
-        uint32_t n_def = 0;
-        rc = readback(&n_def, w.foff + m, 4, s);
-        if (rc) return rc;
-        if (n_def)
+        // Budgeted passes, finest level first: a query whose k-th neighbour
+        // lies beyond +-2 cells of the level is flagged and, compacted and in
+        // sorted order, searched again one level coarser (cells 4x larger).
+        // What is left after the coarsest level goes to the unbudgeted pass,
+        // which picks a level per query.
+        // A deferred query carries the k-th distance found so far (when its
+        // heap was full) to the next pass, which scans only that ball.
+        const float4* list = qs;
+        const float* blist = nullptr;
+        uint32_t n_list = m;
+        float4* next = w.qlist;
+        float* bnext = w.blist;
+        for (int l = 0; l < a.n_levels && n_list; ++l)
         {
-            HIP_TRY(launch_compact_queries(qs, m, w.flags, w.foff, w.qlist, s));
             SearchArgs b = a;
-            b.queries = w.qlist;
-            b.n_query = n_def;
+            b.lv[0] = a.lv[l];
+            b.queries = list;
+            b.n_query = n_list;
+            b.defer_flag = w.flags;
+            b.query_bound = blist;
+            b.defer_bound = w.dbound;
+            HIP_TRY(launch_search(mode, b, hp.block, hp.lds, hp.global, true, s));
+            HIP_TRY(launch_fill_u32(w.flags + n_list, 1, 0u, s));
+            HIP_TRY(launch_scan_exclusive(w.flags, w.foff, (uint64_t)n_list + 1, w.fpart, s));
+            uint32_t n_def = 0;
+            rc = readback(&n_def, w.foff + n_list, 4, s);
+            if (rc) return rc;
+            if (n_def) HIP_TRY(launch_compact_queries(list, n_list, w.flags, w.foff, next, w.dbound, bnext, s));
+            list = next;
+            blist = bnext;
+            n_list = n_def;
+            next = next == w.qlist ? w.qlist2 : w.qlist;
+            bnext = bnext == w.blist ? w.blist2 : w.blist;
+        }
+        if (n_list)
+        {
+            SearchArgs b = a;
+            b.queries = list;
+            b.n_query = n_list;
             b.defer_flag = nullptr;
+            b.query_bound = blist;
+            b.defer_bound = nullptr;
             HIP_TRY(launch_search(mode, b, hp.block, hp.lds, hp.global, false, s));
         }
     }
@@ -659,7 +688,7 @@ int selected_host(const T* xyz, int64_t n, T search_radius, uint32_t max_knn, co
     const size_t b_x = al((size_t)n * 3 * sizeof(T)), b_xf = f64 ? al((size_t)n * 12) : 0,
                  b_o = al((size_t)n * (size_t)n_selected * sizeof(T)), b_f = al((size_t)n_selected * 4),
                  b_h = hp.global ? al((size_t)n * cap * sizeof(T)) + al((size_t)n * cap * 4) : 0;
-    const size_t b_ws = search_ws_bytes((uint64_t)n, (uint64_t)n, 1, false, false);
+    const size_t b_ws = search_ws_bytes((uint64_t)n, (uint64_t)n, 1, false);
     rc = ctx_reserve(b_x + b_xf + b_o + b_f + b_h + b_ws);
     if (rc) return rc;
     Arena ar{t_ctx.dbuf, t_ctx.dbytes, 0};
@@ -673,7 +702,7 @@ int selected_host(const T* xyz, int64_t n, T search_radius, uint32_t max_knn, co
     HIP_TRY(hipMemcpyAsync(d_f, selected, (size_t)n_selected * 4, hipMemcpyHostToDevice, s));
     if (f64) HIP_TRY(launch_f64_to_f32(reinterpret_cast<const double*>(d_x), d_xf, (uint64_t)n * 3, s));
     Arena wa{t_ctx.dbuf + ar.off, t_ctx.dbytes - ar.off, 0};
-    SearchWs w = carve(wa, (uint64_t)n, (uint64_t)n, 1, false, false);
+    SearchWs w = carve(wa, (uint64_t)n, (uint64_t)n, 1, false);
     if (wa.off > wa.size) return fail(PGEOF_ENOMEM, "workspace too small");
     SearchArgs sa{};
     const float4* qs = nullptr;
@@ -775,6 +804,10 @@ int pgeof_set_tuning(const char* key, double v)
         g_tune.grid_levels = v;
     else if (k == "near_first")
         g_tune.near_first = v;
+    else if (k == "budget_cells")
+        g_tune.budget_cells = v;
+    else if (k == "grid_level_ratio")
+        g_tune.grid_level_ratio = v;
     else
         return fail(PGEOF_EINVAL, "unknown tuning key " + k);
     return PGEOF_OK;
@@ -790,6 +823,8 @@ double pgeof_get_tuning(const char* key)
     if (k == "radius_cell_factor") return g_tune.radius_cell_factor;
     if (k == "grid_levels") return g_tune.grid_levels;
     if (k == "near_first") return g_tune.near_first;
+    if (k == "budget_cells") return g_tune.budget_cells;
+    if (k == "grid_level_ratio") return g_tune.grid_level_ratio;
     return -1.0;
 }
 
@@ -855,7 +890,7 @@ int pgeof_knn_search(const float* data, int64_t n_data, const float* query, int6
     const bool self = (query == data && n_query == n_data);
     const size_t b_d = al((size_t)n_data * 12), b_q = self ? 0 : al((size_t)n_query * 12),
                  b_i = al((size_t)n_query * knn * 4), b_o = al((size_t)n_query * knn * 4);
-    const size_t b_ws = search_ws_bytes((uint64_t)n_data, (uint64_t)n_query, knn, false, false);
+    const size_t b_ws = search_ws_bytes((uint64_t)n_data, (uint64_t)n_query, knn, false);
     rc = ctx_reserve(b_d + b_q + b_i + b_o + b_ws);
     if (rc) return rc;
     Arena ar{t_ctx.dbuf, t_ctx.dbytes, 0};
@@ -887,7 +922,7 @@ int pgeof_radius_search(const float* data, int64_t n_data, const float* query, i
     const bool self = (query == data && n_query == n_data);
     const size_t b_d = al((size_t)n_data * 12), b_q = self ? 0 : al((size_t)n_query * 12),
                  b_i = al((size_t)n_query * max_knn * 4), b_o = al((size_t)n_query * max_knn * 4);
-    const size_t b_ws = search_ws_bytes((uint64_t)n_data, (uint64_t)n_query, max_knn, false, false);
+    const size_t b_ws = search_ws_bytes((uint64_t)n_data, (uint64_t)n_query, max_knn, false);
     rc = ctx_reserve(b_d + b_q + b_i + b_o + b_ws);
     if (rc) return rc;
     Arena ar{t_ctx.dbuf, t_ctx.dbytes, 0};
@@ -956,20 +991,18 @@ int pgeof_compute_features_selected_f64(const double* xyz, int64_t n, double sea
 // ---------------------------------------------------- device searches ----
 size_t pgeof_knn_workspace_size(int64_t n_data, int64_t n_query, uint32_t knn)
 {
-    return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), knn, true,
-                           false);
+    return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), knn, false);
 }
 
 size_t pgeof_radius_workspace_size(int64_t n_data, int64_t n_query, uint32_t max_knn)
 {
     return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), max_knn,
-                           false, false);
+                           false);
 }
 
 size_t pgeof_radius_csr_workspace_size(int64_t n, uint32_t max_knn)
 {
-    return search_ws_bytes((uint64_t)std::max<int64_t>(n, 0), (uint64_t)std::max<int64_t>(n, 0), max_knn, false,
-                           true);
+    return search_ws_bytes((uint64_t)std::max<int64_t>(n, 0), (uint64_t)std::max<int64_t>(n, 0), max_knn, true);
 }
 
 int pgeof_knn_search_device(const float* data, int64_t n_data, const float* query, int64_t n_query, uint32_t knn,

@@ -49,9 +49,10 @@ constexpr int kMaxLevels = 4;
 // Arguments of the neighbour-search kernels.
 struct SearchArgs
 {
-    Level lv[kMaxLevels];  // level l has cell edge h_0 * 4^l
+    Level lv[kMaxLevels];  // level l has cell edge h_0 * ratio^l
     const Level* levels;   // the same table in device memory (per-lane level choice reads it)
     int n_levels;
+    const float* data;      // the caller's data points, (n, 3) float32 (fused features gather from it)
     const float4* queries;  // sorted queries (x, y, z, bits of original query index)
     uint32_t n_query;
     uint32_t k;    // kNN: k; radius: max_knn
@@ -62,9 +63,11 @@ struct SearchArgs
     float* out_feat;      // (n_query, 11) or nullptr (fused features)
     uint32_t* out_count;  // per original query: number of hits (radius) or nullptr
     uint64_t* gheap_keys;  // global-memory heap fallback: k * n_query keys
-    uint32_t* gheap_pos;   // k * n_query positions
-    uint32_t* defer_flag;  // budgeted pass: 1 where the query is left to the multi-level pass
+    uint32_t* defer_flag;  // budgeted pass: 1 where the query is left to the next pass
+    const float* query_bound;  // per query: upper bound of its k-th squared distance, or +inf (nullable)
+    float* defer_bound;        // budgeted pass: that bound for the deferred queries
     int near_first;        // visit cell rows nearest-first
+    int budget_cells;      // budgeted pass: largest ball half-width, in cells around the query's cell
 };
 
 // Simple linear device arena carved out of one allocation.
@@ -98,7 +101,7 @@ hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds_by
                          hipStream_t s);
 hipError_t launch_store_levels(const SearchArgs& a, Level* out, hipStream_t s);
 hipError_t launch_compact_queries(const float4* q, uint32_t n, const uint32_t* flag, const uint32_t* off,
-                                  float4* out, hipStream_t s);
+                                  float4* out, const float* bound, float* bound_out, hipStream_t s);
 hipError_t launch_arange_ptr(uint32_t* nn_ptr, uint32_t n_points, uint32_t k, hipStream_t s);
 hipError_t launch_compact_rows(const uint32_t* dense, uint32_t n_rows, uint32_t cap, const uint32_t* counts,
                                const uint32_t* row_ptr, uint32_t* nn, hipStream_t s);

@@ -264,24 +264,22 @@ __global__ __launch_bounds__(256) void k_compact_rows(const uint32_t* __restrict
 }
 
 // --------------------------------------------------------------- heap ----
-// Max-heap of 64-bit keys (squared-distance bits << 32 | original index);
-// element j lives at keys[j * stride].  With POS, the sorted-array position
-// of each entry travels with its key.
-template <bool POS>
+// Keys are 64-bit: squared-distance bits << 32 | original index, so that the
+// order of keys is the order of (distance, index) and ties go to the smaller
+// index.
+
+// Binary max-heap in global memory (large k); element j at keys[j * stride].
 struct Heap
 {
     uint64_t* keys;
-    uint32_t* pos;
     uint32_t stride;
     uint32_t k;
     uint32_t count;  // entries held; the heap property holds once count == k
 
-    __device__ __forceinline__ uint64_t top() const { return keys[0]; }
     __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return keys[i * stride]; }
-    __device__ __forceinline__ uint32_t pos_at(uint32_t i) const { return POS ? pos[i * stride] : 0u; }
 
     // put `key` at index i of heap [0, n) and sift it down
-    __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t p, uint32_t n)
+    __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t n)
     {
         for (;;)
         {
@@ -300,26 +298,22 @@ struct Heap
             }
             if (ck <= key) break;
             keys[i * stride] = ck;
-            if (POS) pos[i * stride] = pos[c * stride];
             i = c;
         }
         keys[i * stride] = key;
-        if (POS) pos[i * stride] = p;
     }
 
     __device__ __forceinline__ void heapify()
     {
-        for (int i = (int)k / 2 - 1; i >= 0; --i)
-            sift_from((uint32_t)i, keys[i * stride], POS ? pos[i * stride] : 0u, k);
+        for (int i = (int)k / 2 - 1; i >= 0; --i) sift_from((uint32_t)i, keys[i * stride], k);
     }
 
     // precondition: key < worst (worst = sentinel until the heap is full)
-    __device__ __forceinline__ void add(uint64_t key, uint32_t p, uint64_t& worst)
+    __device__ __forceinline__ void add(uint64_t key, uint64_t& worst)
     {
         if (count < k)
         {
             keys[count * stride] = key;
-            if (POS) pos[count * stride] = p;
             if (++count == k)
             {
                 heapify();
@@ -328,7 +322,7 @@ struct Heap
         }
         else
         {
-            sift_from(0, key, p, k);
+            sift_from(0, key, k);
             worst = keys[0];
         }
     }
@@ -338,96 +332,81 @@ struct Heap
     {
         if (count < k)
         {
-            for (uint32_t j = count; j < k; ++j)
-            {
-                keys[j * stride] = sentinel;
-                if (POS) pos[j * stride] = 0xFFFFFFFFu;
-            }
+            for (uint32_t j = count; j < k; ++j) keys[j * stride] = sentinel;
             heapify();
         }
         for (uint32_t end = k - 1; end > 0; --end)
         {
             const uint64_t top_k = keys[0];
             const uint64_t last = keys[end * stride];
-            uint32_t tp = 0, lp = 0;
-            if (POS)
-            {
-                tp = pos[0];
-                lp = pos[end * stride];
-            }
             keys[end * stride] = top_k;
-            if (POS) pos[end * stride] = tp;
-            sift_from(0, last, lp, end);
+            sift_from(0, last, end);
         }
     }
 };
 
-// 4-ary max-heap of k 64-bit keys (and positions) for one lane.  Node 0, the
-// largest key, is held in registers; nodes 1..k-1 live in LDS with the four
-// children 4i+1..4i+4 of node i stored as one aligned group of four
-// consecutive slots of this lane, read with two 16-byte loads.  A sift from
-// the root of a 20-entry heap touches two groups (binary heap: four levels).
-template <bool POS>
+// 4-ary max-heap of k keys for one lane, in LDS.  Node 0, the largest key,
+// is held in a register; nodes 1..k-1 live in LDS.  The four children
+// 4i+1..4i+4 of node i form group i, stored as two 16-byte pairs (children
+// 0-1 and 2-3).  Pair p of every lane of the block is contiguous, 16 bytes
+// per lane, so a wave reads one pair with conflict-free 16-byte loads.  A
+// sift from the root of a 20-entry heap touches two groups (binary heap:
+// four levels).
 struct Heap4
 {
-    uint64_t* keys;    // this lane's group 0; group g at keys + g * gstride
-    uint32_t* pos;
-    uint32_t gstride;  // entries between consecutive groups of one lane (4 * block size)
+    uint64_t* keys;    // this lane's pair 0; pair p at keys + p * pstride
+    uint32_t pstride;  // entries between consecutive pairs of one lane (2 * block size)
     uint32_t k;
     uint32_t count;    // entries held; the heap property holds once count == k
     uint64_t rkey;     // node 0
-    uint32_t rpos;
 
-    __device__ __forceinline__ uint32_t slot(uint32_t i) const { return ((i - 1) >> 2) * gstride + ((i - 1) & 3); }
+    __device__ __forceinline__ uint32_t slot(uint32_t i) const
+    {
+        const uint32_t s = i - 1;
+        return (s >> 1) * pstride + (s & 1);
+    }
     __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return i == 0 ? rkey : keys[slot(i)]; }
-    __device__ __forceinline__ uint32_t pos_at(uint32_t i) const { return i == 0 ? rpos : (POS ? pos[slot(i)] : 0u); }
-    __device__ __forceinline__ void put(uint32_t i, uint64_t kk, uint32_t p)
+    __device__ __forceinline__ void put(uint32_t i, uint64_t kk)
     {
         if (i == 0)
-        {
             rkey = kk;
-            rpos = p;
-        }
         else
-        {
             keys[slot(i)] = kk;
-            if (POS) pos[slot(i)] = p;
-        }
     }
 
     // put `key` at node i of the heap of nodes [0, n) and sift it down
-    __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t p, uint32_t n)
+    __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t n)
     {
         for (;;)
         {
             const uint32_t c0 = 4 * i + 1;
             if (c0 >= n) break;
-            const ulonglong2* g = reinterpret_cast<const ulonglong2*>(keys + i * gstride);
-            const ulonglong2 g01 = g[0], g23 = g[1];
+            const ulonglong2 g01 = *reinterpret_cast<const ulonglong2*>(keys + (2 * i) * pstride);
+            const ulonglong2 g23 = *reinterpret_cast<const ulonglong2*>(keys + (2 * i + 1) * pstride);
             uint64_t mk = g01.x;
             uint32_t mj = 0;
             if (c0 + 1 < n && g01.y > mk) mk = g01.y, mj = 1;
             if (c0 + 2 < n && g23.x > mk) mk = g23.x, mj = 2;
             if (c0 + 3 < n && g23.y > mk) mk = g23.y, mj = 3;
             if (mk <= key) break;
-            put(i, mk, POS ? pos[i * gstride + mj] : 0u);
+            put(i, mk);
             i = c0 + mj;
         }
-        put(i, key, p);
+        put(i, key);
     }
 
     __device__ __forceinline__ void heapify()
     {
         if (k < 2) return;
-        for (int i = (int)(k - 2) / 4; i >= 0; --i) sift_from((uint32_t)i, key_at(i), pos_at(i), k);
+        for (int i = (int)(k - 2) / 4; i >= 0; --i) sift_from((uint32_t)i, key_at(i), k);
     }
 
     // precondition: key < worst (worst = sentinel until the heap is full)
-    __device__ __forceinline__ void add(uint64_t key, uint32_t p, uint64_t& worst)
+    __device__ __forceinline__ void add(uint64_t key, uint64_t& worst)
     {
         if (count < k)
         {
-            put(count, key, p);
+            put(count, key);
             if (++count == k)
             {
                 heapify();
@@ -436,7 +415,7 @@ struct Heap4
         }
         else
         {
-            sift_from(0, key, p, k);
+            sift_from(0, key, k);
             worst = rkey;
         }
     }
@@ -446,16 +425,14 @@ struct Heap4
     {
         if (count < k)
         {
-            for (uint32_t j = count; j < k; ++j) put(j, sentinel, 0xFFFFFFFFu);
+            for (uint32_t j = count; j < k; ++j) put(j, sentinel);
             heapify();
         }
         for (uint32_t end = k - 1; end > 0; --end)
         {
             const uint64_t last = keys[slot(end)];
-            const uint32_t lp = POS ? pos[slot(end)] : 0u;
             keys[slot(end)] = rkey;
-            if (POS) pos[slot(end)] = rpos;
-            sift_from(0, last, lp, end);
+            sift_from(0, last, end);
         }
     }
 };
@@ -531,7 +508,6 @@ __device__ __forceinline__ Level pick_level(const SearchArgs& a, int l) { return
 template <int MODE, int WAVES, bool GLOBAL_HEAP, bool BUDGET>
 __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
 {
-    constexpr bool POS = (MODE == kModeKnnFeat);
     constexpr bool RADIUS = (MODE == kModeRadius || MODE == kModeRadiusCsr);
     constexpr int BLOCK = WAVES * 64;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -542,27 +518,21 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     float4* stage = reinterpret_cast<float4*>(smem) + wid * kStage;
     unsigned char* hbase = smem + (size_t)WAVES * kStage * sizeof(float4);
 
-    // LDS heaps: 4-ary with the root in registers; global-memory fallback for
-    // large k: binary
-    using HeapT = typename std::conditional<GLOBAL_HEAP, Heap<POS>, Heap4<POS>>::type;
+    // LDS heaps: 4-ary with the root in a register (ceil((k - 1) / 4) groups
+    // of four keys per lane); global-memory fallback for large k: binary
+    using HeapT = typename std::conditional<GLOBAL_HEAP, Heap, Heap4>::type;
     HeapT heap;
     if constexpr (GLOBAL_HEAP)
     {
         // inactive lanes never touch their (non-existent) global heap
         heap.keys = a.gheap_keys + (active ? t : 0);
-        heap.pos = a.gheap_pos + (active ? t : 0);
         heap.stride = a.n_query;
     }
     else
     {
-        // ceil((k - 1) / 4) groups of four keys (and four positions) per lane
-        const uint32_t groups = (a.k + 2) / 4;
-        heap.keys = reinterpret_cast<uint64_t*>(hbase) + (size_t)threadIdx.x * 4;
-        heap.pos = reinterpret_cast<uint32_t*>(hbase + (size_t)groups * BLOCK * 4 * sizeof(uint64_t)) +
-                   (size_t)threadIdx.x * 4;
-        heap.gstride = 4 * BLOCK;
+        heap.keys = reinterpret_cast<uint64_t*>(hbase) + (size_t)threadIdx.x * 2;
+        heap.pstride = 2 * BLOCK;
         heap.rkey = 0;
-        heap.rpos = 0;
     }
     heap.k = a.k;
     heap.count = 0;
@@ -573,13 +543,21 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     uint64_t worst = sentinel;
 
     bool live = active && isfinite(q4.x) && isfinite(q4.y) && isfinite(q4.z);
+    // known upper bound of the k-th squared distance (from an earlier pass), or +inf
+    const float qbound = (!RADIUS && a.query_bound && active) ? a.query_bound[t] : INFINITY;
+    const bool bounded = qbound < INFINITY;
     bool deferred = false;  // BUDGET: query handed over to the unbudgeted pass
     const float qxf = q4.x, qyf = q4.y, qzf = q4.z;
     const double qx = qxf, qy = qyf, qz = qzf;
 
     // ---- level: finest one whose 3x3x3 cube holds k points ----
     int level = 0;
-    if (!RADIUS && !BUDGET && live)
+    if (!RADIUS && !BUDGET && live && bounded)
+    {
+        const double rb = sqrt((double)qbound) * 1.00001;
+        while (level < a.n_levels - 1 && rb > 2.0 * pick_level(a, level).h) ++level;
+    }
+    else if (!RADIUS && !BUDGET && live)
     {
         for (; level < a.n_levels - 1; ++level)
         {
@@ -608,9 +586,11 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     const int cz = live ? cell_coord(qz, G.lo[2], G.inv_h, G2) : 0;
     const double h = G.h, margin = G.h * (double)a.margin;
 
+    // squared pruning distance: the current k-th distance, or the known
+    // bound while the heap is not full, with slack for rounding
     auto wlim = [&]() -> double {
-        if (worst == ~0ull) return INFINITY;
-        const float w = __uint_as_float((uint32_t)(worst >> 32));
+        float w = worst == ~0ull ? INFINITY : __uint_as_float((uint32_t)(worst >> 32));
+        w = fminf(w, qbound);
         if (!(w < INFINITY)) return INFINITY;
         return (double)w * 1.00001 + 1e-37;
     };
@@ -618,7 +598,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     auto consider = [&](const float4& p, uint32_t j) {
         const float d2 = sqdist32(qxf, qyf, qzf, p.x, p.y, p.z);
         const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
-        if (key < worst) heap.add(key, j, worst);
+        if (key < worst) heap.add(key, worst);
     };
 
 
@@ -782,7 +762,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     bool covered = false;
     if (!RADIUS)
     {
-        bool need = live;
+        // a query with a known bound goes straight to the ball of that bound
+        bool need = live && !bounded;
         for (int n = BUDGET ? 1 : 0; __ballot(need) != 0; ++n)
         {
             const Box B = need ? cube_box(cx, cy, cz, n, G) : empty_box();
@@ -796,7 +777,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                 if (worst != sentinel || covered) need = false;
                 if (BUDGET && need && n >= 1)
                 {
-                    // sparse neighbourhood: leave it to the multi-level pass
+                    // sparse neighbourhood: leave it to the next pass
                     need = false;
                     deferred = true;
                     live = false;
@@ -821,8 +802,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
             }
             else
                 B = Box{0, G0 - 1, 0, G1 - 1, 0, G2 - 1};
-            if (BUDGET && (B.x0 < cx - 2 || B.x1 > cx + 2 || B.y0 < cy - 2 || B.y1 > cy + 2 || B.z0 < cz - 2 ||
-                           B.z1 > cz + 2))
+            const int bc = a.budget_cells;
+            if (BUDGET && (B.x0 < cx - bc || B.x1 > cx + bc || B.y0 < cy - bc || B.y1 > cy + bc ||
+                           B.z0 < cz - bc || B.z1 > cz + bc))
             {
                 B = empty_box();
                 deferred = true;
@@ -837,7 +819,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     if (BUDGET)
     {
         a.defer_flag[t] = deferred ? 1u : 0u;
-        if (deferred) return;
+        if (deferred)
+        {
+            // the k-th distance so far (heap full) bounds the final one
+            const float wd = worst == ~0ull ? INFINITY : __uint_as_float((uint32_t)(worst >> 32));
+            a.defer_bound[t] = fminf(wd, qbound);
+            return;
+        }
     }
     heap.finish(sentinel);
     const uint32_t k = a.k;
@@ -896,18 +884,34 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         float f[11];
         if (live)
         {
+            // neighbour coordinates are gathered from the caller's xyz by
+            // original index, four at a time so the loads overlap
             Moments m;
             mom_zero(m);
-            double ox = 0.0, oy = 0.0, oz = 0.0;
-            for (uint32_t j = 0; j < k; ++j)
+            const float* __restrict__ xyz = a.data;
+            const uint32_t i0 = (uint32_t)heap.key_at(0);
+            const double ox = xyz[(size_t)i0 * 3 + 0], oy = xyz[(size_t)i0 * 3 + 1], oz = xyz[(size_t)i0 * 3 + 2];
+            constexpr uint32_t kChunk = 4;
+            for (uint32_t j0 = 0; j0 < k; j0 += kChunk)
             {
-                const uint64_t key = heap.key_at(j);
-                const uint32_t p = heap.pos_at(j);
-                const float4 pt = pts[p];
-                if (j == 0) ox = pt.x, oy = pt.y, oz = pt.z;
-                mom_add(m, (double)pt.x - ox, (double)pt.y - oy, (double)pt.z - oz);
-                oi[j] = (uint32_t)key;
-                if (od) od[j] = __uint_as_float((uint32_t)(key >> 32));
+                float px[kChunk], py[kChunk], pz[kChunk];
+#pragma unroll
+                for (uint32_t u = 0; u < kChunk; ++u)
+                {
+                    const uint64_t key = heap.key_at(j0 + u < k ? j0 + u : 0);
+                    const uint32_t o = (uint32_t)key;
+                    px[u] = xyz[(size_t)o * 3 + 0];
+                    py[u] = xyz[(size_t)o * 3 + 1];
+                    pz[u] = xyz[(size_t)o * 3 + 2];
+                    if (j0 + u < k)
+                    {
+                        oi[j0 + u] = o;
+                        if (od) od[j0 + u] = __uint_as_float((uint32_t)(key >> 32));
+                    }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kChunk; ++u)
+                    if (j0 + u < k) mom_add(m, (double)px[u] - ox, (double)py[u] - oy, (double)pz[u] - oz);
             }
             double c[6];
             mom_cov(m, (double)k, c);
@@ -1210,10 +1214,15 @@ hipError_t launch_search_m(const SearchArgs& a, int block, size_t lds, bool gh, 
 
 __global__ __launch_bounds__(256) void k_compact_queries(const float4* __restrict__ q, uint32_t n,
                                                          const uint32_t* __restrict__ flag,
-                                                         const uint32_t* __restrict__ off, float4* __restrict__ out)
+                                                         const uint32_t* __restrict__ off, float4* __restrict__ out,
+                                                         const float* __restrict__ bound, float* __restrict__ bound_out)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n && flag[i]) out[off[i]] = q[i];
+    if (i < n && flag[i])
+    {
+        out[off[i]] = q[i];
+        if (bound) bound_out[off[i]] = bound[i];
+    }
 }
 
 // Histogram for grids with few cells: per-block counts in LDS, one global
@@ -1321,10 +1330,10 @@ hipError_t launch_store_levels(const SearchArgs& a, Level* out, hipStream_t s)
 }
 
 hipError_t launch_compact_queries(const float4* q, uint32_t n, const uint32_t* flag, const uint32_t* off,
-                                  float4* out, hipStream_t s)
+                                  float4* out, const float* bound, float* bound_out, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    k_compact_queries<<<(n + 255) / 256, 256, 0, s>>>(q, n, flag, off, out);
+    k_compact_queries<<<(n + 255) / 256, 256, 0, s>>>(q, n, flag, off, out, bound, bound_out);
     return hipGetLastError();
 }
 

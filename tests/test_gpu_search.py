@@ -153,7 +153,7 @@ def test_knn_sparse_outliers_use_coarse_levels():
     _eq_knn(xyz, xyz, 20, kdtree=False)
 
 
-@pytest.mark.parametrize("levels", [1, 2])
+@pytest.mark.parametrize("levels", [1, 2, 3, 4])
 def test_knn_level_count_does_not_change_results(levels):
     from point_geometric_features_amd import capi
 
@@ -162,11 +162,20 @@ def test_knn_level_count_does_not_change_results(levels):
     try:
         capi.set_tuning("grid_levels", levels)
         gi, gd = pgeof.knn_search(xyz, xyz, 20)
+        x = torch.from_numpy(xyz).cuda()
+        nn_d = torch.empty(len(xyz) * 20, dtype=torch.int32, device="cuda")
+        ptr_d = torch.empty(len(xyz) + 1, dtype=torch.int32, device="cuda")
+        f_d = torch.empty((len(xyz), 11), dtype=torch.float32, device="cuda")
+        capi.knn_features_device(x, 20, nn_d, ptr_d, f_d)
+        torch.cuda.synchronize()
     finally:
         capi.set_tuning("grid_levels", old)
     oi, od = O.knn(xyz, xyz, 20, kdtree=True)
     np.testing.assert_array_equal(gi, oi)
     np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(nn_d.cpu().numpy().view(np.uint32), gi.reshape(-1))
+    nn, ptr = scenes.knn_csr(gi)
+    np.testing.assert_array_equal(f_d.cpu().numpy(), pgeof.compute_features(xyz, nn, ptr, 20))
 
 
 @pytest.mark.parametrize("near", [0, 1])
