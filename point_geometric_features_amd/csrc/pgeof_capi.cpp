@@ -279,12 +279,12 @@ HeapPlan heap_plan_bytes(uint32_t k, size_t entry_bytes)
 }
 
 // Search kernels: per wave 64 heaps plus a 128-point staging buffer (2 KiB)
-// in LDS.  A heap keeps its root in a register and nodes 1..k-1 in groups of
-// four slots: ceil((k - 1) / 4) groups of four 8-byte keys per lane.
+// in LDS.  A heap keeps nodes 0..4 in registers and nodes 5..k-1 in groups of
+// four slots: ceil((k - 5) / 4) groups of four 8-byte keys per lane.
 HeapPlan heap_plan(uint32_t k)
 {
     const Tuning t = tuning();
-    const size_t per = (size_t)4 * ((k + 2) / 4) * 8;
+    const size_t per = (size_t)4 * (k > 5 ? (k - 2) / 4 : 0) * 8;
     const size_t stage = 128 * 16;
     const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
     for (int b : {256, 128, 64})

@@ -345,13 +345,14 @@ struct Heap
     }
 };
 
-// 4-ary max-heap of k keys for one lane, in LDS.  Node 0, the largest key,
-// is held in a register; nodes 1..k-1 live in LDS.  The four children
-// 4i+1..4i+4 of node i form group i, stored as two 16-byte pairs (children
-// 0-1 and 2-3).  Pair p of every lane of the block is contiguous, 16 bytes
-// per lane, so a wave reads one pair with conflict-free 16-byte loads.  A
-// sift from the root of a 20-entry heap touches two groups (binary heap:
-// four levels).
+// 4-ary max-heap of k keys for one lane.  Node 0 (the largest key) and its
+// four children, nodes 1..4, are held in registers; nodes 5..k-1 live in
+// LDS.  The four children 4i+1..4i+4 of node i >= 1 form LDS group i - 1,
+// stored as two 16-byte pairs (children 0-1 and 2-3).  Pair p of every lane
+// of the block is contiguous, 16 bytes per lane, so a wave reads one pair with
+// conflict-free 16-byte loads.  A sift from the root of a 20-entry heap reads
+// four registers and one LDS group; the binary-heap equivalent walks four
+// levels of LDS.
 struct Heap4
 {
     uint64_t* keys;    // this lane's pair 0; pair p at keys + p * pstride
@@ -359,30 +360,57 @@ struct Heap4
     uint32_t k;
     uint32_t count;    // entries held; the heap property holds once count == k
     uint64_t rkey;     // node 0
+    uint64_t n1, n2, n3, n4;  // nodes 1..4 (separate scalars, never an indexed array)
 
     __device__ __forceinline__ uint32_t slot(uint32_t i) const
     {
-        const uint32_t s = i - 1;
+        const uint32_t s = i - 5;
         return (s >> 1) * pstride + (s & 1);
     }
-    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return i == 0 ? rkey : keys[slot(i)]; }
+    // The register nodes are combined with bit masks, never indexed or
+    // stored in a branch: a select chain over consecutive members would be
+    // turned back into an indexed access, which lives in scratch memory.
+    __device__ __forceinline__ static uint64_t mask(bool b) { return 0ull - (uint64_t)b; }
+    __device__ __forceinline__ uint64_t key_at(uint32_t i) const
+    {
+        const uint64_t r = (rkey & mask(i == 0)) | (n1 & mask(i == 1)) | (n2 & mask(i == 2)) |
+                           (n3 & mask(i == 3)) | (n4 & mask(i == 4));
+        return i <= 4 ? r : keys[slot(i)];
+    }
     __device__ __forceinline__ void put(uint32_t i, uint64_t kk)
     {
-        if (i == 0)
-            rkey = kk;
-        else
-            keys[slot(i)] = kk;
+        const uint64_t m0 = mask(i == 0), m1 = mask(i == 1), m2 = mask(i == 2), m3 = mask(i == 3), m4 = mask(i == 4);
+        rkey = (kk & m0) | (rkey & ~m0);
+        n1 = (kk & m1) | (n1 & ~m1);
+        n2 = (kk & m2) | (n2 & ~m2);
+        n3 = (kk & m3) | (n3 & ~m3);
+        n4 = (kk & m4) | (n4 & ~m4);
+        if (i > 4) keys[slot(i)] = kk;
     }
 
     // put `key` at node i of the heap of nodes [0, n) and sift it down
     __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t n)
     {
+        if (i == 0)
+        {
+            // children of the root are registers
+            uint64_t mk = n > 1 ? n1 : 0;
+            uint32_t mj = 0;
+            if (2 < n && n2 > mk) mk = n2, mj = 1;
+            if (3 < n && n3 > mk) mk = n3, mj = 2;
+            if (4 < n && n4 > mk) mk = n4, mj = 3;
+            const bool up = n > 1 && mk > key;
+            rkey = up ? mk : key;
+            if (!up) return;
+            i = 1 + mj;
+        }
         for (;;)
         {
             const uint32_t c0 = 4 * i + 1;
             if (c0 >= n) break;
-            const ulonglong2 g01 = *reinterpret_cast<const ulonglong2*>(keys + (2 * i) * pstride);
-            const ulonglong2 g23 = *reinterpret_cast<const ulonglong2*>(keys + (2 * i + 1) * pstride);
+            const uint32_t g = i - 1;
+            const ulonglong2 g01 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g) * pstride);
+            const ulonglong2 g23 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g + 1) * pstride);
             uint64_t mk = g01.x;
             uint32_t mj = 0;
             if (c0 + 1 < n && g01.y > mk) mk = g01.y, mj = 1;
@@ -430,8 +458,8 @@ struct Heap4
         }
         for (uint32_t end = k - 1; end > 0; --end)
         {
-            const uint64_t last = keys[slot(end)];
-            keys[slot(end)] = rkey;
+            const uint64_t last = key_at(end);
+            put(end, rkey);
             sift_from(0, last, end);
         }
     }
@@ -505,6 +533,11 @@ __device__ __forceinline__ int near_index(int i, int c, int lo, int hi)
 // per-lane value would move the whole argument block to scratch memory.
 __device__ __forceinline__ Level pick_level(const SearchArgs& a, int l) { return a.levels[l]; }
 
+#ifndef PGEOF_SCAN_UNROLL
+#define PGEOF_SCAN_UNROLL 4
+#endif
+constexpr uint32_t kScanUnroll = PGEOF_SCAN_UNROLL;  // candidates loaded together per lane
+
 template <int MODE, int WAVES, bool GLOBAL_HEAP, bool BUDGET>
 __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
 {
@@ -518,8 +551,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     float4* stage = reinterpret_cast<float4*>(smem) + wid * kStage;
     unsigned char* hbase = smem + (size_t)WAVES * kStage * sizeof(float4);
 
-    // LDS heaps: 4-ary with the root in a register (ceil((k - 1) / 4) groups
-    // of four keys per lane); global-memory fallback for large k: binary
+    // heaps: 4-ary with nodes 0..4 in registers and ceil((k - 5) / 4) groups
+    // of four keys per lane in LDS; global-memory fallback for large k: binary
     using HeapT = typename std::conditional<GLOBAL_HEAP, Heap, Heap4>::type;
     HeapT heap;
     if constexpr (GLOBAL_HEAP)
@@ -533,6 +566,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         heap.keys = reinterpret_cast<uint64_t*>(hbase) + (size_t)threadIdx.x * 2;
         heap.pstride = 2 * BLOCK;
         heap.rkey = 0;
+        heap.n1 = heap.n2 = heap.n3 = heap.n4 = 0;
     }
     heap.k = a.k;
     heap.count = 0;
@@ -602,12 +636,24 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     };
 
 
+    // Candidates are loaded kScanUnroll at a time (indices clamped to the
+    // range, so every load is in bounds) and then considered in order: the
+    // loads overlap instead of each waiting for the previous one.
+
     // per-lane scan of sorted positions [s0, e0) and [s1, e1) from global memory
     auto scan_direct = [&](uint32_t s0, uint32_t e0, uint32_t s1, uint32_t e1) {
         for (int pc = 0; pc < 2; ++pc)
         {
             const uint32_t lo = pc ? s1 : s0, hi = pc ? e1 : e0;
-            for (uint32_t j = lo; j < hi; ++j) consider(pts[j], j);
+            for (uint32_t j = lo; j < hi; j += kScanUnroll)
+            {
+                float4 p[kScanUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = pts[min(j + u, hi - 1)];
+#pragma unroll
+                for (uint32_t u = 0; u < kScanUnroll; ++u)
+                    if (j + u < hi) consider(p[u], j + u);
+            }
         }
     };
 
@@ -616,7 +662,15 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         for (int pc = 0; pc < 2; ++pc)
         {
             const uint32_t lo = max(pc ? s1 : s0, c), hi = min(pc ? e1 : e0, c + cnt);
-            for (uint32_t j = lo; j < hi; ++j) consider(stage[j - c], j);
+            for (uint32_t j = lo; j < hi; j += kScanUnroll)
+            {
+                float4 p[kScanUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = stage[min(j + u, hi - 1) - c];
+#pragma unroll
+                for (uint32_t u = 0; u < kScanUnroll; ++u)
+                    if (j + u < hi) consider(p[u], j + u);
+            }
         }
     };
 
