@@ -530,6 +530,10 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
         }
         if (n_list)
         {
+            // slowest queries first, lanes of similar cost in each wave
+            HIP_TRY(launch_order_by_bound(list, blist, n_list, w.count, w.flags, w.foff, w.scan_part, next, bnext, s));
+            list = next;
+            blist = bnext;
             SearchArgs b = a;
             b.queries = list;
             b.n_query = n_list;

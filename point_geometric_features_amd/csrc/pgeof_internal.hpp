@@ -100,6 +100,10 @@ hipError_t launch_scatter(const float* xyz, uint32_t n, const uint32_t* cell, co
 hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds_bytes, bool global_heap, bool budget,
                          hipStream_t s);
 hipError_t launch_store_levels(const SearchArgs& a, Level* out, hipStream_t s);
+// reorder queries (and their bounds) by bound, largest first; count needs
+// 2049 entries, key and rank n
+hipError_t launch_order_by_bound(const float4* q, const float* bound, uint32_t n, uint32_t* count, uint32_t* key,
+                                 uint32_t* rank, uint32_t* scan_part, float4* q_out, float* bound_out, hipStream_t s);
 hipError_t launch_compact_queries(const float4* q, uint32_t n, const uint32_t* flag, const uint32_t* off,
                                   float4* out, const float* bound, float* bound_out, hipStream_t s);
 hipError_t launch_arange_ptr(uint32_t* nn_ptr, uint32_t n_points, uint32_t k, hipStream_t s);

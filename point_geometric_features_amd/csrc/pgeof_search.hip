@@ -1367,6 +1367,88 @@ hipError_t launch_hist(const float* xyz, uint32_t n, const GridParams& gp, uint3
     return hipGetLastError();
 }
 
+// ---- work order of the deferred queries ----
+// A deferred query's search cost grows with its bound of the k-th distance,
+// and a wave runs as long as its slowest lane: the unbudgeted pass therefore
+// takes the deferred queries ordered by bound, largest first (+inf, unknown
+// bound, first of all).  Bucket = the bound's float bits >> 20 (exponent and
+// three mantissa bits), reversed.
+constexpr uint32_t kBoundBuckets = 2048;
+
+__device__ __forceinline__ uint32_t bound_bucket(float b)
+{
+    const uint32_t bits = (__float_as_uint(b) & 0x7FFFFFFFu) >> 20;
+    return (kBoundBuckets - 1) - min(bits, kBoundBuckets - 1);
+}
+
+__global__ __launch_bounds__(256) void k_bound_hist(const float* __restrict__ bound, uint32_t n,
+                                                    uint32_t* __restrict__ count, uint32_t* __restrict__ key,
+                                                    uint32_t* __restrict__ rank)
+{
+    __shared__ uint32_t bins[kBoundBuckets];
+    for (uint32_t c = threadIdx.x; c < kBoundBuckets; c += 256) bins[c] = 0;
+    __syncthreads();
+    constexpr uint32_t kPer = 16;
+    const uint64_t base = (uint64_t)blockIdx.x * 256 * kPer;
+    uint32_t b[kPer], lr[kPer];
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u)
+    {
+        const uint64_t i = base + (uint64_t)u * 256 + threadIdx.x;
+        b[u] = 0;
+        lr[u] = 0;
+        if (i < n)
+        {
+            b[u] = bound_bucket(bound[i]);
+            lr[u] = atomicAdd(&bins[b[u]], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < kBoundBuckets; c += 256)
+    {
+        const uint32_t v = bins[c];
+        bins[c] = v ? atomicAdd(&count[c], v) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u)
+    {
+        const uint64_t i = base + (uint64_t)u * 256 + threadIdx.x;
+        if (i < n)
+        {
+            key[i] = b[u];
+            rank[i] = bins[b[u]] + lr[u];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bound_scatter(const float4* __restrict__ q, const float* __restrict__ bound,
+                                                       uint32_t n, const uint32_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ rank,
+                                                       const uint32_t* __restrict__ start, float4* __restrict__ q_out,
+                                                       float* __restrict__ bound_out)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = start[key[i]] + rank[i];
+    q_out[p] = q[i];
+    bound_out[p] = bound[i];
+}
+
+hipError_t launch_order_by_bound(const float4* q, const float* bound, uint32_t n, uint32_t* count, uint32_t* key,
+                                 uint32_t* rank, uint32_t* scan_part, float4* q_out, float* bound_out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    ProfScope ps("knn_defer_order", s);
+    hipError_t e = launch_fill_u32(count, kBoundBuckets + 1, 0u, s);
+    if (e != hipSuccess) return e;
+    k_bound_hist<<<(n + 256 * 16 - 1) / (256 * 16), 256, 0, s>>>(bound, n, count, key, rank);
+    e = launch_scan_exclusive(count, count, kBoundBuckets + 1, scan_part, s);
+    if (e != hipSuccess) return e;
+    k_bound_scatter<<<(n + 255) / 256, 256, 0, s>>>(q, bound, n, key, rank, count, q_out, bound_out);
+    return hipGetLastError();
+}
+
 __global__ void k_store_levels(SearchArgs a, Level* out)
 {
     if (threadIdx.x != 0) return;
