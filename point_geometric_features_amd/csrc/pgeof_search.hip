@@ -371,13 +371,12 @@ struct Heap4
     // stored in a branch: a select chain over consecutive members would be
     // turned back into an indexed access, which lives in scratch memory.
     __device__ __forceinline__ static uint64_t mask(bool b) { return 0ull - (uint64_t)b; }
-    __device__ __forceinline__ uint64_t key_at(uint32_t i) const
+    __device__ __forceinline__ uint64_t reg(uint32_t i) const  // i in 0..4
     {
-        const uint64_t r = (rkey & mask(i == 0)) | (n1 & mask(i == 1)) | (n2 & mask(i == 2)) |
-                           (n3 & mask(i == 3)) | (n4 & mask(i == 4));
-        return i <= 4 ? r : keys[slot(i)];
+        return (rkey & mask(i == 0)) | (n1 & mask(i == 1)) | (n2 & mask(i == 2)) | (n3 & mask(i == 3)) |
+               (n4 & mask(i == 4));
     }
-    __device__ __forceinline__ void put(uint32_t i, uint64_t kk)
+    __device__ __forceinline__ void set_reg(uint32_t i, uint64_t kk)  // i in 0..4
     {
         const uint64_t m0 = mask(i == 0), m1 = mask(i == 1), m2 = mask(i == 2), m3 = mask(i == 3), m4 = mask(i == 4);
         rkey = (kk & m0) | (rkey & ~m0);
@@ -385,15 +384,37 @@ struct Heap4
         n2 = (kk & m2) | (n2 & ~m2);
         n3 = (kk & m3) | (n3 & ~m3);
         n4 = (kk & m4) | (n4 & ~m4);
-        if (i > 4) keys[slot(i)] = kk;
+    }
+    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return i <= 4 ? reg(i) : keys[slot(i)]; }
+    __device__ __forceinline__ void put(uint32_t i, uint64_t kk)
+    {
+        if (i <= 4)
+            set_reg(i, kk);
+        else
+            keys[slot(i)] = kk;
     }
 
-    // put `key` at node i of the heap of nodes [0, n) and sift it down
+    // largest of the children 4i+1..4i+4 (< n) of node i >= 1, which are LDS group i - 1
+    __device__ __forceinline__ uint64_t max_child(uint32_t i, uint32_t n, uint32_t& c) const
+    {
+        const uint32_t c0 = 4 * i + 1, g = i - 1;
+        const ulonglong2 g01 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g) * pstride);
+        const ulonglong2 g23 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g + 1) * pstride);
+        uint64_t mk = g01.x;
+        uint32_t mj = 0;
+        if (c0 + 1 < n && g01.y > mk) mk = g01.y, mj = 1;
+        if (c0 + 2 < n && g23.x > mk) mk = g23.x, mj = 2;
+        if (c0 + 3 < n && g23.y > mk) mk = g23.y, mj = 3;
+        c = c0 + mj;
+        return mk;
+    }
+
+    // put `key` at node i of the heap of nodes [0, n) and sift it down:
+    // node 0, then one of nodes 1..4 (registers), then LDS nodes only
     __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t n)
     {
         if (i == 0)
         {
-            // children of the root are registers
             uint64_t mk = n > 1 ? n1 : 0;
             uint32_t mj = 0;
             if (2 < n && n2 > mk) mk = n2, mj = 1;
@@ -404,23 +425,25 @@ struct Heap4
             if (!up) return;
             i = 1 + mj;
         }
+        if (i <= 4)
+        {
+            uint32_t c = 0;
+            const uint64_t mk = 4 * i + 1 < n ? max_child(i, n, c) : 0;
+            const bool down = 4 * i + 1 < n && mk > key;
+            set_reg(i, down ? mk : key);
+            if (!down) return;
+            i = c;
+        }
         for (;;)
         {
-            const uint32_t c0 = 4 * i + 1;
-            if (c0 >= n) break;
-            const uint32_t g = i - 1;
-            const ulonglong2 g01 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g) * pstride);
-            const ulonglong2 g23 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g + 1) * pstride);
-            uint64_t mk = g01.x;
-            uint32_t mj = 0;
-            if (c0 + 1 < n && g01.y > mk) mk = g01.y, mj = 1;
-            if (c0 + 2 < n && g23.x > mk) mk = g23.x, mj = 2;
-            if (c0 + 3 < n && g23.y > mk) mk = g23.y, mj = 3;
+            if (4 * i + 1 >= n) break;
+            uint32_t c;
+            const uint64_t mk = max_child(i, n, c);
             if (mk <= key) break;
-            put(i, mk);
-            i = c0 + mj;
+            keys[slot(i)] = mk;
+            i = c;
         }
-        put(i, key);
+        keys[slot(i)] = key;
     }
 
     __device__ __forceinline__ void heapify()
@@ -456,10 +479,17 @@ struct Heap4
             for (uint32_t j = count; j < k; ++j) put(j, sentinel);
             heapify();
         }
-        for (uint32_t end = k - 1; end > 0; --end)
+        uint32_t end = k - 1;
+        for (; end > 4; --end)
         {
-            const uint64_t last = key_at(end);
-            put(end, rkey);
+            const uint64_t last = keys[slot(end)];
+            keys[slot(end)] = rkey;
+            sift_from(0, last, end);
+        }
+        for (; end > 0; --end)
+        {
+            const uint64_t last = reg(end);
+            set_reg(end, rkey);
             sift_from(0, last, end);
         }
     }
