@@ -188,7 +188,11 @@ void pgeof_server_destroy(pgeof_server* server);
  * (depth of the kNN grid pyramid, 1..4), "near_first" (1: cell rows are
  * visited nearest-first, 0: in index order), "budget_cells" (budgeted kNN
  * pass: a query whose ball reaches farther than this many cells from its own
- * cell is deferred to the next pass; 1..64), "grid_level_ratio" (cell edge
+ * cell is deferred to the next pass; 1..64), "budget_shells" (budgeted kNN
+ * pass: cubic shells around the query's cell scanned before a query that has
+ * not found k candidates is deferred; 1..budget_cells), "budget_levels"
+ * (pyramid levels, finest first, that get a budgeted pass; coarser levels
+ * serve the unbudgeted pass only), "grid_level_ratio" (cell edge
  * ratio between consecutive pyramid levels, 2..8).
  * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);

@@ -58,6 +58,8 @@ struct Tuning
     double grid_levels = 1.0;           // kNN grid pyramid depth (1..4)
     double near_first = 1.0;            // visit rows nearest-first (1) or in index order (0)
     double budget_cells = 2.0;          // budgeted kNN pass: ball half-width limit in cells
+    double budget_shells = 1.0;         // budgeted kNN pass: cubic shells scanned before deferring
+    double budget_levels = 4.0;         // pyramid levels that get a budgeted pass (the rest: unbudgeted only)
     double grid_level_ratio = 2.0;      // cell edge ratio between consecutive pyramid levels (2..8)
 };
 Tuning g_tune;
@@ -486,6 +488,7 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     a.r2 = radius * radius;
     a.near_first = tuning().near_first > 0.5 ? 1 : 0;
     a.budget_cells = std::max(1, std::min(64, (int)tuning().budget_cells));
+    a.budget_shells = std::max(1, std::min(a.budget_cells, (int)tuning().budget_shells));
     a.out_idx = csr ? (void*)w.dense : out_idx;
     a.out_d2 = out_d2;
     a.out_feat = out_feat;
@@ -506,7 +509,8 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
         uint32_t n_list = m;
         float4* next = w.qlist;
         float* bnext = w.blist;
-        for (int l = 0; l < a.n_levels && n_list; ++l)
+        const int budget_levels = std::max(1, std::min(a.n_levels, (int)tuning().budget_levels));
+        for (int l = 0; l < budget_levels && n_list; ++l)
         {
             SearchArgs b = a;
             b.lv[0] = a.lv[l];
@@ -810,6 +814,10 @@ int pgeof_set_tuning(const char* key, double v)
         g_tune.near_first = v;
     else if (k == "budget_cells")
         g_tune.budget_cells = v;
+    else if (k == "budget_shells")
+        g_tune.budget_shells = v;
+    else if (k == "budget_levels")
+        g_tune.budget_levels = v;
     else if (k == "grid_level_ratio")
         g_tune.grid_level_ratio = v;
     else
@@ -828,6 +836,8 @@ double pgeof_get_tuning(const char* key)
     if (k == "grid_levels") return g_tune.grid_levels;
     if (k == "near_first") return g_tune.near_first;
     if (k == "budget_cells") return g_tune.budget_cells;
+    if (k == "budget_shells") return g_tune.budget_shells;
+    if (k == "budget_levels") return g_tune.budget_levels;
     if (k == "grid_level_ratio") return g_tune.grid_level_ratio;
     return -1.0;
 }

@@ -68,6 +68,7 @@ struct SearchArgs
     float* defer_bound;        // budgeted pass: that bound for the deferred queries
     int near_first;        // visit cell rows nearest-first
     int budget_cells;      // budgeted pass: largest ball half-width, in cells around the query's cell
+    int budget_shells;     // budgeted pass: cubic shells scanned before a query without k candidates is deferred
 };
 
 // Simple linear device arena carved out of one allocation.

@@ -859,7 +859,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                 covered = (cx - n <= 0) && (cx + n >= G0 - 1) && (cy - n <= 0) && (cy + n >= G1 - 1) &&
                           (cz - n <= 0) && (cz + n >= G2 - 1);
                 if (worst != sentinel || covered) need = false;
-                if (BUDGET && need && n >= 1)
+                if (BUDGET && need && n >= a.budget_shells)
                 {
                     // sparse neighbourhood: leave it to the next pass
                     need = false;
