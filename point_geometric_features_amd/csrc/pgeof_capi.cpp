@@ -51,15 +51,15 @@ int hip_fail(hipError_t e, const char* where)
 // ------------------------------------------------------------- tuning ----
 struct Tuning
 {
-    double knn_occupancy = 0.08;       // target points per cell = max(0.75, this * k)
-    double grid_cells_per_point = 4.0;  // cap on G / n
+    double knn_occupancy = 0.35;       // target points per cell = max(0.75, this * k)
+    double grid_cells_per_point = 8.0;  // cap on G / n
     double lds_budget = 65536.0;        // bytes of LDS per workgroup for heaps
     double radius_cell_factor = 0.5;    // cell edge = this * radius
-    double grid_levels = 1.0;           // kNN grid pyramid depth (1..4)
+    double grid_levels = 3.0;           // kNN grid pyramid depth (1..4)
     double near_first = 1.0;            // visit rows nearest-first (1) or in index order (0)
     double budget_cells = 2.0;          // budgeted kNN pass: ball half-width limit in cells
     double budget_shells = 1.0;         // budgeted kNN pass: cubic shells scanned before deferring
-    double budget_levels = 4.0;         // pyramid levels that get a budgeted pass (the rest: unbudgeted only)
+    double budget_levels = 1.0;         // pyramid levels that get a budgeted pass (the rest: unbudgeted only)
     double grid_level_ratio = 2.0;      // cell edge ratio between consecutive pyramid levels (2..8)
 };
 Tuning g_tune;
@@ -427,9 +427,15 @@ int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, ui
         uint32_t* csl = w.cs_all + cs_used;
         cs_used += gp.G + 1;
         HIP_TRY(launch_fill_u32(w.count, gp.G + 1, 0u, s));
-        HIP_TRY(launch_hist(data, n, gp, w.count, w.cell, w.rank, s));
+        if (l == 0)
+            HIP_TRY(launch_hist(data, n, gp, w.count, w.cell, w.rank, s));
+        else  // from the level-0 order: runs of equal cells, one atomic each
+            HIP_TRY(launch_hist_sorted(w.pts[0], n, gp, w.count, w.cell, w.rank, s));
         HIP_TRY(launch_scan_exclusive(w.count, csl, gp.G + 1, w.scan_part, s));
-        HIP_TRY(launch_scatter(data, n, w.cell, w.rank, csl, w.pts[l], s));
+        if (l == 0)
+            HIP_TRY(launch_scatter(data, n, w.cell, w.rank, csl, w.pts[l], s));
+        else
+            HIP_TRY(launch_scatter_sorted(w.pts[0], n, w.cell, w.rank, csl, w.pts[l], s));
         Level& L = sa.lv[l];
         for (int a = 0; a < 3; ++a)
         {

@@ -131,6 +131,57 @@ __global__ __launch_bounds__(256) void k_hist(const float* __restrict__ xyz, uin
     if (rank) rank[i] = r;
 }
 
+// Histogram of points that are already sorted by a finer grid (the level-0
+// array): neighbouring points mostly share a cell, so each run of equal cells
+// within a wave takes one atomic for the whole run.  rank = the run's base +
+// the lane's offset in the run; cell/rank are indexed by sorted position.
+__global__ __launch_bounds__(256) void k_hist_sorted(const float4* __restrict__ pts, uint32_t n, GridParams gp,
+                                                     uint32_t* __restrict__ count, uint32_t* __restrict__ cell,
+                                                     uint32_t* __restrict__ rank)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool active = i < n;
+    uint32_t cid = 0xFFFFFFFFu;
+    if (active)
+    {
+        const float4 p = pts[i];
+        const int cx = cell_coord((double)p.x, gp.lo[0], gp.inv_h, gp.g[0]);
+        const int cy = cell_coord((double)p.y, gp.lo[1], gp.inv_h, gp.g[1]);
+        const int cz = cell_coord((double)p.z, gp.lo[2], gp.inv_h, gp.g[2]);
+        cid = ((uint32_t)cz * (uint32_t)gp.g[1] + (uint32_t)cy) * (uint32_t)gp.g[0] + (uint32_t)cx;
+    }
+    const uint32_t prev = __shfl_up(cid, 1);
+    const bool head = active && (lane == 0 || prev != cid);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long act = __ballot(active);
+    // run of this lane: [h, e)
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int h = 63 - __clzll(heads & upto);
+    const unsigned long long above = heads & ~upto;
+    const int n_act = __popcll(act);
+    const int e = above ? min(__ffsll(above) - 1, n_act) : n_act;
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&count[cid], (uint32_t)(e - h));
+    base = __shfl(base, h);
+    if (active)
+    {
+        cell[i] = cid;
+        rank[i] = base + (uint32_t)(lane - h);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter_sorted(const float4* __restrict__ in, uint32_t n,
+                                                        const uint32_t* __restrict__ cell,
+                                                        const uint32_t* __restrict__ rank,
+                                                        const uint32_t* __restrict__ cell_start,
+                                                        float4* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[cell_start[cell[i]] + rank[i]] = in[i];
+}
+
 __global__ __launch_bounds__(256) void k_sumsq(const uint32_t* __restrict__ count, uint64_t G,
                                                unsigned long long* __restrict__ out)
 {
@@ -1519,6 +1570,24 @@ hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, 
     k_scan_reduce<<<nt, kScanBlock, 0, s>>>(in, n, partials);
     k_scan_partials<<<1, kScanBlock, 0, s>>>(partials, nt);
     k_scan_apply<<<nt, kScanBlock, 0, s>>>(in, out, n, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist_sorted(const float4* pts, uint32_t n, const GridParams& gp, uint32_t* count, uint32_t* cell,
+                              uint32_t* rank, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    ProfScope ps("grid_histogram_sorted", s);
+    k_hist_sorted<<<(n + 255) / 256, 256, 0, s>>>(pts, n, gp, count, cell, rank);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_sorted(const float4* in, uint32_t n, const uint32_t* cell, const uint32_t* rank,
+                                 const uint32_t* cell_start, float4* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    ProfScope ps("grid_scatter", s);
+    k_scatter_sorted<<<(n + 255) / 256, 256, 0, s>>>(in, n, cell, rank, cell_start, out);
     return hipGetLastError();
 }
 
