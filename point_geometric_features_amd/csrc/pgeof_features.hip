@@ -87,22 +87,24 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
         return;
     }
 
-    // validate the row first, so the walk below has no early exit and its
-    // gathers can be issued ahead of their use
-    bool bad = false;
-    for (uint32_t j = 0; j < walk; ++j) bad |= row[j] >= a.n_xyz;
-    if (bad)
+    // One pass over the row: indices are checked as they are read (a bad one
+    // is replaced by 0 so every gather stays in bounds, and the row is zeroed
+    // at the end); coordinates are gathered as one 12-byte load each, four
+    // neighbours at a time.
+    if (a.n_xyz == 0)  // every index is out of range and there is nothing to gather
     {
         atomicOr(a.err, kErrNnIndex);
         for (uint32_t c = 0; c < n_out; ++c) out[c] = 0.0f;
         return;
     }
-
+    const float3* __restrict__ xyz3 = reinterpret_cast<const float3*>(xyz);
+    bool bad = row[0] >= a.n_xyz;
+    const uint32_t i0 = bad ? 0u : row[0];
+    const float3 p0 = xyz3[i0];
+    const double ox = p0.x, oy = p0.y, oz = p0.z;
     Moments m, best;
     mom_zero(m);
     mom_zero(best);
-    const uint32_t i0 = row[0];
-    const double ox = xyz[(size_t)i0 * 3 + 0], oy = xyz[(size_t)i0 * 3 + 1], oz = xyz[(size_t)i0 * 3 + 2];
     uint32_t s = 0;
     float h_best = 1.0f;
     uint32_t k_opt = k_nn;
@@ -113,10 +115,13 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
 #pragma unroll
         for (uint32_t u = 0; u < kChunk; ++u)
         {
-            const uint32_t idx = (j0 + u < walk) ? row[j0 + u] : i0;
-            px[u] = xyz[(size_t)idx * 3 + 0];
-            py[u] = xyz[(size_t)idx * 3 + 1];
-            pz[u] = xyz[(size_t)idx * 3 + 2];
+            uint32_t idx = (j0 + u < walk) ? row[j0 + u] : i0;
+            bad |= idx >= a.n_xyz;
+            idx = idx < a.n_xyz ? idx : 0u;
+            const float3 p = xyz3[idx];
+            px[u] = p.x;
+            py[u] = p.y;
+            pz[u] = p.z;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kChunk; ++u)
@@ -151,6 +156,12 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
                 }
             }
         }
+    }
+    if (bad)
+    {
+        atomicOr(a.err, kErrNnIndex);
+        for (uint32_t c = 0; c < n_out; ++c) out[c] = 0.0f;
+        return;
     }
     if (MODE == kCsrPlain)
     {

@@ -1023,9 +1023,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
             // original index, four at a time so the loads overlap
             Moments m;
             mom_zero(m);
-            const float* __restrict__ xyz = a.data;
-            const uint32_t i0 = (uint32_t)heap.key_at(0);
-            const double ox = xyz[(size_t)i0 * 3 + 0], oy = xyz[(size_t)i0 * 3 + 1], oz = xyz[(size_t)i0 * 3 + 2];
+            const float3* __restrict__ xyz3 = reinterpret_cast<const float3*>(a.data);
+            const float3 p0 = xyz3[(uint32_t)heap.key_at(0)];
+            const double ox = p0.x, oy = p0.y, oz = p0.z;
             constexpr uint32_t kChunk = 4;
             for (uint32_t j0 = 0; j0 < k; j0 += kChunk)
             {
@@ -1035,9 +1035,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
                 {
                     const uint64_t key = heap.key_at(j0 + u < k ? j0 + u : 0);
                     const uint32_t o = (uint32_t)key;
-                    px[u] = xyz[(size_t)o * 3 + 0];
-                    py[u] = xyz[(size_t)o * 3 + 1];
-                    pz[u] = xyz[(size_t)o * 3 + 2];
+                    const float3 p = xyz3[o];  // one 12-byte load
+                    px[u] = p.x;
+                    py[u] = p.y;
+                    pz[u] = p.z;
                     if (j0 + u < k)
                     {
                         oi[j0 + u] = o;

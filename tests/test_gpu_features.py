@@ -110,6 +110,14 @@ def test_features_invalid_csr_raises():
         pgeof.compute_features(xyz, nn[:3], np.array([0, 3], np.uint32), k_min=0)
     with pytest.raises(TypeError):
         pgeof.compute_features(xyz, nn.astype(np.int64), ptr)
+    # an index past the end in the middle of a row, for every CSR driver
+    with pytest.raises(ValueError):
+        pgeof.compute_features_multiscale(xyz, nn, ptr, [1, 2])
+    with pytest.raises(ValueError):
+        pgeof.compute_features_optimal(xyz, nn, ptr)
+    # rows that point into an empty cloud
+    with pytest.raises(ValueError):
+        pgeof.compute_features(np.zeros((0, 3), np.float32), nn[:2], np.array([0, 2], np.uint32))
 
 
 def test_multiscale_reference_test():
