@@ -409,7 +409,6 @@ struct Heap4
     uint64_t* keys;    // this lane's pair 0; pair p at keys + p * pstride
     uint32_t pstride;  // entries between consecutive pairs of one lane (2 * block size)
     uint32_t k;
-    uint32_t count;    // entries held; the heap property holds once count == k
     uint64_t rkey;     // node 0
     uint64_t n1, n2, n3, n4;  // nodes 1..4 (separate scalars, never an indexed array)
 
@@ -497,39 +496,29 @@ struct Heap4
         keys[slot(i)] = key;
     }
 
-    __device__ __forceinline__ void heapify()
+    // Start full of the sentinel (larger than or equal to every key that can
+    // be accepted): the heap is valid from the start, so every insertion is
+    // the same replace-the-root sift, wherever each lane is in its scan; no
+    // lane switches from filling to sifting at its own moment.
+    __device__ __forceinline__ void init(uint64_t sentinel)
     {
-        if (k < 2) return;
-        for (int i = (int)(k - 2) / 4; i >= 0; --i) sift_from((uint32_t)i, key_at(i), k);
+        rkey = n1 = n2 = n3 = n4 = sentinel;
+        const ulonglong2 ss = make_ulonglong2(sentinel, sentinel);
+        for (uint32_t p = 0; 4 * p < (k > 5 ? 2 * (k - 5) : 0); ++p)
+            *reinterpret_cast<ulonglong2*>(keys + p * pstride) = ss;
     }
 
-    // precondition: key < worst (worst = sentinel until the heap is full)
+    // precondition: key < worst, worst = the root (the sentinel until k keys
+    // are in)
     __device__ __forceinline__ void add(uint64_t key, uint64_t& worst)
     {
-        if (count < k)
-        {
-            put(count, key);
-            if (++count == k)
-            {
-                heapify();
-                worst = rkey;
-            }
-        }
-        else
-        {
-            sift_from(0, key, k);
-            worst = rkey;
-        }
+        sift_from(0, key, k);
+        worst = rkey;
     }
 
-    // pad with the sentinel, restore the heap property, sort ascending
-    __device__ __forceinline__ void finish(uint64_t sentinel)
+    // sort ascending (entries never filled keep the sentinel)
+    __device__ __forceinline__ void finish(uint64_t)
     {
-        if (count < k)
-        {
-            for (uint32_t j = count; j < k; ++j) put(j, sentinel);
-            heapify();
-        }
         uint32_t end = k - 1;
         for (; end > 4; --end)
         {
@@ -646,16 +635,17 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
     {
         heap.keys = reinterpret_cast<uint64_t*>(hbase) + (size_t)threadIdx.x * 2;
         heap.pstride = 2 * BLOCK;
-        heap.rkey = 0;
-        heap.n1 = heap.n2 = heap.n3 = heap.n4 = 0;
     }
     heap.k = a.k;
-    heap.count = 0;
 
     const float4 q4 = active ? a.queries[t] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t qorig = __float_as_uint(q4.w);
     const uint64_t sentinel = RADIUS ? ((uint64_t)__float_as_uint(a.r2) << 32) : ~0ull;
     uint64_t worst = sentinel;
+    if constexpr (GLOBAL_HEAP)
+        heap.count = 0;
+    else
+        heap.init(sentinel);
 
     bool live = active && isfinite(q4.x) && isfinite(q4.y) && isfinite(q4.z);
     // known upper bound of the k-th squared distance (from an earlier pass), or +inf
