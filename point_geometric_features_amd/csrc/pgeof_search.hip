@@ -893,7 +893,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         {
             const Box B = need ? cube_box(cx, cy, cz, n, G) : empty_box();
             const Box C = (need && n > (BUDGET ? 1 : 0)) ? cube_box(cx, cy, cz, n - 1, G) : empty_box();
-            scan_pass(B, C, false);
+            scan_pass(B, C, true);
             if (need)
             {
                 n_done = n;
