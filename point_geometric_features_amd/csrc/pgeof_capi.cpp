@@ -421,11 +421,11 @@ int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, ui
     {
         const double hl = h * std::pow(std::max(2.0, std::min(8.0, t.grid_level_ratio)), (double)l);
         const GridParams gp = make_grid(bb, hl);
-        if (l > 0 && (cs_used + gp.G + 1 > cs_room || sa.lv[l - 1].g[0] * (uint64_t)sa.lv[l - 1].g[1] *
+        if (l > 0 && (cs_used + ((gp.G + 1 + 3) & ~uint64_t(3)) > cs_room || sa.lv[l - 1].g[0] * (uint64_t)sa.lv[l - 1].g[1] *
                                                               (uint64_t)sa.lv[l - 1].g[2] <= 64))
             break;
         uint32_t* csl = w.cs_all + cs_used;
-        cs_used += gp.G + 1;
+        cs_used += (gp.G + 1 + 3) & ~uint64_t(3);  // keep every level's table 16-byte aligned
         HIP_TRY(launch_fill_u32(w.count, gp.G + 1, 0u, s));
         if (l == 0)
             HIP_TRY(launch_hist(data, n, gp, w.count, w.cell, w.rank, s));

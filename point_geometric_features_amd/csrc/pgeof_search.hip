@@ -228,15 +228,31 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& t
     return wprefix + inc - v;
 }
 
+// Tiles of kScanTile entries, read as 16-byte vectors: one pass of a block
+// covers 4 * kScanBlock consecutive entries, four per thread, so every
+// wave-wide load is contiguous.  VEC requires 16-byte aligned arrays.
+template <bool VEC>
+__device__ __forceinline__ uint4 scan_load4(const uint32_t* in, uint64_t i, uint64_t n)
+{
+    if (VEC && i + 3 < n) return reinterpret_cast<const uint4*>(in)[i >> 2];
+    uint4 v;
+    v.x = i < n ? in[i] : 0u;
+    v.y = i + 1 < n ? in[i + 1] : 0u;
+    v.z = i + 2 < n ? in[i + 2] : 0u;
+    v.w = i + 3 < n ? in[i + 3] : 0u;
+    return v;
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
                                                             uint32_t* __restrict__ partials)
 {
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     uint32_t s = 0;
-    for (int j = 0; j < kScanItems; ++j)
+    for (int sub = 0; sub < kScanTile; sub += 4 * kScanBlock)
     {
-        const uint64_t i = base + (uint64_t)j * kScanBlock + threadIdx.x;
-        if (i < n) s += in[i];
+        const uint4 v = scan_load4<VEC>(in, base + sub + 4 * threadIdx.x, n);
+        s += v.x + v.y + v.z + v.w;
     }
     uint32_t total;
     block_exclusive_scan(s, total);
@@ -268,25 +284,29 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t* __restri
     }
 }
 
+template <bool VEC>
 __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* in, uint32_t* out, uint64_t n,
                                                            const uint32_t* __restrict__ partials)
 {
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t loc = 0;
-    for (int j = 0; j < kScanItems; ++j)
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint32_t carry = partials[blockIdx.x];
+    for (int sub = 0; sub < kScanTile; sub += 4 * kScanBlock)
     {
-        const uint64_t i = base + j;
-        v[j] = i < n ? in[i] : 0u;
-        loc += v[j];
-    }
-    uint32_t total;
-    uint32_t ex = block_exclusive_scan(loc, total) + partials[blockIdx.x];
-    for (int j = 0; j < kScanItems; ++j)
-    {
-        const uint64_t i = base + j;
-        if (i < n) out[i] = ex;
-        ex += v[j];
+        const uint64_t i = base + sub + 4 * threadIdx.x;
+        const uint4 v = scan_load4<VEC>(in, i, n);  // read before the block scan syncs: in may alias out
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v.x + v.y + v.z + v.w, total) + carry;
+        const uint4 o = make_uint4(ex, ex + v.x, ex + v.x + v.y, ex + v.x + v.y + v.z);
+        if (VEC && i + 3 < n)
+            reinterpret_cast<uint4*>(out)[i >> 2] = o;
+        else
+        {
+            if (i < n) out[i] = o.x;
+            if (i + 1 < n) out[i + 1] = o.y;
+            if (i + 2 < n) out[i + 2] = o.z;
+            if (i + 3 < n) out[i + 3] = o.w;
+        }
+        carry += total;
     }
 }
 
@@ -1558,9 +1578,16 @@ hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, 
     if (n == 0) return hipSuccess;
     ProfScope ps("grid_scan", s);
     const uint32_t nt = (uint32_t)((n + kScanTile - 1) / kScanTile);
-    k_scan_reduce<<<nt, kScanBlock, 0, s>>>(in, n, partials);
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    if (vec)
+        k_scan_reduce<true><<<nt, kScanBlock, 0, s>>>(in, n, partials);
+    else
+        k_scan_reduce<false><<<nt, kScanBlock, 0, s>>>(in, n, partials);
     k_scan_partials<<<1, kScanBlock, 0, s>>>(partials, nt);
-    k_scan_apply<<<nt, kScanBlock, 0, s>>>(in, out, n, partials);
+    if (vec)
+        k_scan_apply<true><<<nt, kScanBlock, 0, s>>>(in, out, n, partials);
+    else
+        k_scan_apply<false><<<nt, kScanBlock, 0, s>>>(in, out, n, partials);
     return hipGetLastError();
 }
 
