@@ -805,7 +805,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         const int my_rows = has ? (B.z1 - B.z0 + 1) * (B.y1 - B.y0 + 1) : 0;
         const int max_rows = wave_max_i(my_rows);
         const long union_rows = (long)(Z1 - Z0 + 1) * (long)(Y1 - Y0 + 1);
-        const bool coop = (lmin == lmax) && union_rows <= 2L * max_rows + 8;
+        // The unbudgeted kNN pass always scans per lane: its queries are
+        // sparse and far apart, so shared staging costs more than it saves
+        // (measured).
+        const bool coop = (RADIUS || BUDGET) && (lmin == lmax) && union_rows <= 2L * max_rows + 8;
         if (coop)
         {
             // every lane stages from the working level's array, also lanes
