@@ -400,14 +400,19 @@ int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, ui
         double h1 = std::cbrt(vol * T / std::max<double>(n, 1.0));
         h1 = std::max(h1, hcap);
         GridParams g1 = make_grid(bb, h1);
+        // point-weighted occupancy at h1, estimated from every stride-th
+        // point: with sampling fraction p, E[sum c_s^2] = p^2 sum c^2 +
+        // p (1 - p) n
+        const uint32_t stride = std::max<uint32_t>(1, std::min<uint32_t>(64, n >> 20));
+        const double ns = (double)((n + stride - 1) / stride), pf = 1.0 / stride;
         HIP_TRY(launch_fill_u32(w.count, g1.G, 0u, s));
-        HIP_TRY(launch_hist(data, n, g1, w.count, nullptr, nullptr, s));
+        HIP_TRY(launch_hist_probe(data, n, stride, g1, w.count, s));
         HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sumsq), 2, 0u, s));
         HIP_TRY(launch_sumsq(w.count, g1.G, w.sumsq, s));
         unsigned long long ss = 0;
         rc = readback(&ss, w.sumsq, sizeof ss, s);
         if (rc) return rc;
-        const double m1 = (double)ss / std::max<double>(n, 1.0);
+        const double m1 = std::max(1.0, ((double)ss - (1.0 - pf) * ns) / (pf * std::max(ns, 1.0)));
         h = h1;
         if (m1 > T) h = h1 * std::sqrt(T / m1);
         h = std::max(h, hcap);

@@ -101,6 +101,9 @@ hipError_t launch_scatter(const float* xyz, uint32_t n, const uint32_t* cell, co
 hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds_bytes, bool global_heap, bool budget,
                          hipStream_t s);
 hipError_t launch_store_levels(const SearchArgs& a, Level* out, hipStream_t s);
+// cell counts of every stride-th point (grid sizing estimate)
+hipError_t launch_hist_probe(const float* xyz, uint32_t n, uint32_t stride, const GridParams& gp, uint32_t* count,
+                             hipStream_t s);
 // coarser grid levels from the level-0 array (already sorted by a finer grid)
 hipError_t launch_hist_sorted(const float4* pts, uint32_t n, const GridParams& gp, uint32_t* count, uint32_t* cell,
                               uint32_t* rank, hipStream_t s);

@@ -182,6 +182,18 @@ __global__ __launch_bounds__(256) void k_scatter_sorted(const float4* __restrict
     out[cell_start[cell[i]] + rank[i]] = in[i];
 }
 
+// Cell counts of every `stride`-th point (grid sizing estimate only).
+__global__ __launch_bounds__(256) void k_hist_probe(const float* __restrict__ xyz, uint32_t n, uint32_t stride,
+                                                    GridParams gp, uint32_t* __restrict__ count)
+{
+    const uint64_t i = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * stride;
+    if (i >= n) return;
+    const int cx = cell_coord((double)xyz[i * 3 + 0], gp.lo[0], gp.inv_h, gp.g[0]);
+    const int cy = cell_coord((double)xyz[i * 3 + 1], gp.lo[1], gp.inv_h, gp.g[1]);
+    const int cz = cell_coord((double)xyz[i * 3 + 2], gp.lo[2], gp.inv_h, gp.g[2]);
+    atomicAdd(&count[((uint32_t)cz * (uint32_t)gp.g[1] + (uint32_t)cy) * (uint32_t)gp.g[0] + (uint32_t)cx], 1u);
+}
+
 __global__ __launch_bounds__(256) void k_sumsq(const uint32_t* __restrict__ count, uint64_t G,
                                                unsigned long long* __restrict__ out)
 {
@@ -1591,6 +1603,16 @@ hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, 
         k_scan_apply<true><<<nt, kScanBlock, 0, s>>>(in, out, n, partials);
     else
         k_scan_apply<false><<<nt, kScanBlock, 0, s>>>(in, out, n, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist_probe(const float* xyz, uint32_t n, uint32_t stride, const GridParams& gp, uint32_t* count,
+                             hipStream_t s)
+{
+    const uint32_t ns = (n + stride - 1) / stride;
+    if (ns == 0) return hipSuccess;
+    ProfScope ps("grid_probe", s);
+    k_hist_probe<<<(ns + 255) / 256, 256, 0, s>>>(xyz, n, stride, gp, count);
     return hipGetLastError();
 }
 
