@@ -1044,35 +1044,69 @@ __global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
         float f[11];
         if (live)
         {
-            // neighbour coordinates are gathered from the caller's xyz by
-            // original index, four at a time so the loads overlap
+            // Neighbour coordinates are gathered from the caller's xyz by
+            // original index (one 12-byte load each), four at a time, the next
+            // four issued before the current ones are accumulated.  Rows of
+            // the index (and distance) output are written four entries per
+            // store when k is a multiple of 4 and the arrays are 16-byte
+            // aligned (every row is then aligned too).
             Moments m;
             mom_zero(m);
             const float3* __restrict__ xyz3 = reinterpret_cast<const float3*>(a.data);
-            const float3 p0 = xyz3[(uint32_t)heap.key_at(0)];
-            const double ox = p0.x, oy = p0.y, oz = p0.z;
             constexpr uint32_t kChunk = 4;
+            const bool vec_out =
+                (k & 3u) == 0 && ((reinterpret_cast<uintptr_t>(a.out_idx) | reinterpret_cast<uintptr_t>(a.out_d2)) & 15u) == 0;
+            uint64_t kn[kChunk];
+            float3 pn[kChunk];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; ++u)
+            {
+                kn[u] = heap.key_at(u < k ? u : 0);
+                pn[u] = xyz3[(uint32_t)kn[u]];
+            }
+            const double ox = pn[0].x, oy = pn[0].y, oz = pn[0].z;
             for (uint32_t j0 = 0; j0 < k; j0 += kChunk)
             {
-                float px[kChunk], py[kChunk], pz[kChunk];
+                uint64_t kc[kChunk];
+                float3 pc[kChunk];
 #pragma unroll
                 for (uint32_t u = 0; u < kChunk; ++u)
                 {
-                    const uint64_t key = heap.key_at(j0 + u < k ? j0 + u : 0);
-                    const uint32_t o = (uint32_t)key;
-                    const float3 p = xyz3[o];  // one 12-byte load
-                    px[u] = p.x;
-                    py[u] = p.y;
-                    pz[u] = p.z;
-                    if (j0 + u < k)
+                    kc[u] = kn[u];
+                    pc[u] = pn[u];
+                }
+                if (j0 + kChunk < k)
+                {
+#pragma unroll
+                    for (uint32_t u = 0; u < kChunk; ++u)
                     {
-                        oi[j0 + u] = o;
-                        if (od) od[j0 + u] = __uint_as_float((uint32_t)(key >> 32));
+                        const uint32_t j = j0 + kChunk + u;
+                        kn[u] = heap.key_at(j < k ? j : 0);
+                        pn[u] = xyz3[(uint32_t)kn[u]];
                     }
+                }
+                if (vec_out)
+                {
+                    *reinterpret_cast<uint4*>(oi + j0) =
+                        make_uint4((uint32_t)kc[0], (uint32_t)kc[1], (uint32_t)kc[2], (uint32_t)kc[3]);
+                    if (od)
+                        *reinterpret_cast<float4*>(od + j0) = make_float4(
+                            __uint_as_float((uint32_t)(kc[0] >> 32)), __uint_as_float((uint32_t)(kc[1] >> 32)),
+                            __uint_as_float((uint32_t)(kc[2] >> 32)), __uint_as_float((uint32_t)(kc[3] >> 32)));
+                }
+                else
+                {
+#pragma unroll
+                    for (uint32_t u = 0; u < kChunk; ++u)
+                        if (j0 + u < k)
+                        {
+                            oi[j0 + u] = (uint32_t)kc[u];
+                            if (od) od[j0 + u] = __uint_as_float((uint32_t)(kc[u] >> 32));
+                        }
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < kChunk; ++u)
-                    if (j0 + u < k) mom_add(m, (double)px[u] - ox, (double)py[u] - oy, (double)pz[u] - oz);
+                    if (j0 + u < k) mom_add(m, (double)pc[u].x - ox, (double)pc[u].y - oy, (double)pc[u].z - oz);
             }
             double c[6];
             mom_cov(m, (double)k, c);
