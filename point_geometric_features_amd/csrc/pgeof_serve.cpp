@@ -10,8 +10,8 @@
 //   sum of the footprints of the requests in flight plus its own fits the
 //   budget; a request larger than the whole budget is rejected at submit.
 // - The grid and the CSR stay in the slot's device memory; the features (and
-//   the CSR when the caller asks for it) are copied back through the slot's
-//   pinned staging buffer.
+//   the CSR when the caller asks for it) are copied straight into the
+//   caller's arrays.
 // - Each request's result does not depend on what else is in flight: every
 //   kernel is deterministic and a slot's buffers are private to it.
 #include <hip/hip_runtime.h>
@@ -62,8 +62,6 @@ struct Slot
     hipStream_t stream = nullptr;
     char* dev = nullptr;
     size_t dev_bytes = 0;
-    char* pin = nullptr;
-    size_t pin_bytes = 0;
 };
 
 }  // namespace
@@ -88,7 +86,7 @@ struct pgeof_server
 namespace
 {
 
-int ensure(Slot& s, size_t dev_bytes, size_t pin_bytes)
+int ensure(Slot& s, size_t dev_bytes)
 {
     if (dev_bytes > s.dev_bytes)
     {
@@ -97,14 +95,6 @@ int ensure(Slot& s, size_t dev_bytes, size_t pin_bytes)
         s.dev_bytes = 0;
         if (hipMalloc(&s.dev, dev_bytes) != hipSuccess) return PGEOF_ENOMEM;
         s.dev_bytes = dev_bytes;
-    }
-    if (pin_bytes > s.pin_bytes)
-    {
-        if (s.pin) (void)hipHostFree(s.pin);
-        s.pin = nullptr;
-        s.pin_bytes = 0;
-        if (hipHostMalloc(&s.pin, pin_bytes, hipHostMallocDefault) != hipSuccess) return PGEOF_ENOMEM;
-        s.pin_bytes = pin_bytes;
     }
     return PGEOF_OK;
 }
@@ -118,8 +108,7 @@ int run_request(pgeof_server* S, Slot& slot, const Request& r)
     if (n == 0) return PGEOF_OK;
     const size_t b_x = al(n * 12), b_nn = al(n * k * 4), b_ptr = al((n + 1) * 4), b_f = al(n * 11 * 4);
     const size_t b_ws = pgeof_knn_workspace_size(r.n, r.n, k);
-    const size_t pin = std::max(b_x, b_f + (r.nn_out ? b_nn : 0));
-    int rc = ensure(slot, b_x + b_nn + b_ptr + b_f + b_ws + 4096, pin);
+    int rc = ensure(slot, b_x + b_nn + b_ptr + b_f + b_ws + 4096);
     if (rc) return rc;
     char* d = slot.dev;
     float* d_x = (float*)d;
@@ -128,17 +117,16 @@ int run_request(pgeof_server* S, Slot& slot, const Request& r)
     float* d_f = (float*)(d + b_x + b_nn + b_ptr);
     void* d_ws = d + b_x + b_nn + b_ptr + b_f;
     hipStream_t st = slot.stream;
-    std::memcpy(slot.pin, r.xyz, n * 12);
-    if (hipMemcpyAsync(d_x, slot.pin, n * 12, hipMemcpyHostToDevice, st) != hipSuccess) return PGEOF_ERUNTIME;
+    // straight from and into the caller's memory: the runtime stages pageable
+    // memory itself, pipelined (measured faster than a pinned buffer per slot
+    // plus host copies)
+    if (hipMemcpyAsync(d_x, r.xyz, n * 12, hipMemcpyHostToDevice, st) != hipSuccess) return PGEOF_ERUNTIME;
     rc = pgeof_knn_features_device(d_x, r.n, k, d_nn, d_ptr, nullptr, d_f, d_ws, b_ws, st);
     if (rc) return rc;
-    if (hipMemcpyAsync(slot.pin, d_f, n * 11 * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return PGEOF_ERUNTIME;
-    if (r.nn_out &&
-        hipMemcpyAsync(slot.pin + b_f, d_nn, n * k * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+    if (hipMemcpyAsync(r.feat_out, d_f, n * 11 * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return PGEOF_ERUNTIME;
+    if (r.nn_out && hipMemcpyAsync(r.nn_out, d_nn, n * k * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
         return PGEOF_ERUNTIME;
     if (hipStreamSynchronize(st) != hipSuccess) return PGEOF_ERUNTIME;
-    std::memcpy(r.feat_out, slot.pin, n * 11 * 4);
-    if (r.nn_out) std::memcpy(r.nn_out, slot.pin + b_f, n * k * 4);
     return PGEOF_OK;
 }
 
@@ -244,7 +232,6 @@ void pgeof_server_destroy(pgeof_server* S)
     {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         if (s.dev) (void)hipFree(s.dev);
-        if (s.pin) (void)hipHostFree(s.pin);
         if (s.stream) (void)hipStreamDestroy(s.stream);
     }
     delete S;
