@@ -43,7 +43,7 @@ def test_knn_uniform_10k_k20_bitexact():
     assert (np.diff(gd, axis=1) >= 0).all()
 
 
-@pytest.mark.parametrize("k", [1, 2, 7, 33, 64, 100])
+@pytest.mark.parametrize("k", [1, 2, 4, 5, 6, 7, 20, 21, 33, 64, 100])
 def test_knn_various_k(k):
     xyz = scenes.uniform_cube(3000, seed=k)
     _eq_knn(xyz, xyz, k, kdtree=False)
@@ -207,3 +207,50 @@ def test_knn_scan_order_does_not_change_results(near):
     np.testing.assert_array_equal(rd, od)
     nn, ptr = scenes.knn_csr(gi)
     np.testing.assert_array_equal(fused, pgeof.compute_features(xyz, nn, ptr, 20))
+
+
+_KNOB_SETS = [
+    {"budget_cells": 1},
+    {"budget_cells": 3, "budget_shells": 3},
+    {"grid_levels": 4, "budget_levels": 4},
+    {"grid_levels": 3, "grid_level_ratio": 4},
+    {"grid_cells_per_point": 1, "grid_levels": 1},
+    {"grid_cells_per_point": 32, "knn_occupancy": 0.05},
+    {"knn_occupancy": 2.0},
+    {"lds_budget": 16384},
+    {"lds_budget": 4096},  # heaps in global memory
+]
+
+
+@pytest.mark.parametrize("knobs", _KNOB_SETS, ids=lambda d: ",".join(f"{k}={v}" for k, v in d.items()))
+def test_tuning_knobs_change_speed_only(knobs):
+    """Every knob setting gives the oracle's neighbours bit for bit, and the
+    fused path the two-call path's features bit for bit."""
+    from point_geometric_features_amd import capi
+
+    xyz = scenes.lidar_scene(50_000, seed=29, extent=20.0)
+    old = {k: capi.get_tuning(k) for k in knobs}
+    try:
+        for k, v in knobs.items():
+            capi.set_tuning(k, v)
+        gi, gd = pgeof.knn_search(xyz, xyz, 16)
+        x = torch.from_numpy(xyz).cuda()
+        nn_d = torch.empty(len(xyz) * 16, dtype=torch.int32, device="cuda")
+        ptr_d = torch.empty(len(xyz) + 1, dtype=torch.int32, device="cuda")
+        f_d = torch.empty((len(xyz), 11), dtype=torch.float32, device="cuda")
+        ws = torch.empty(capi.knn_workspace_size(len(xyz), len(xyz), 16), dtype=torch.uint8, device="cuda")
+        capi.knn_features_device(x, 16, nn_d, ptr_d, f_d, workspace=ws)
+        torch.cuda.synchronize()
+        ri, rd = pgeof.radius_search(xyz, xyz, 0.3, 24)
+    finally:
+        for k, v in old.items():
+            capi.set_tuning(k, v)
+    oi, od = O.knn(xyz, xyz, 16, kdtree=True)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(nn_d.cpu().numpy().view(np.uint32), gi.reshape(-1))
+    nn, ptr = scenes.knn_csr(gi)
+    np.testing.assert_array_equal(f_d.cpu().numpy(), pgeof.compute_features(xyz, nn, ptr))
+    oi, od, _ = O.radius(xyz, xyz, 0.3, 24, kdtree=True)
+    np.testing.assert_array_equal(ri, oi)
+    np.testing.assert_array_equal(rd, od)
