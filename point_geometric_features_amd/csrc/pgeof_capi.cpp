@@ -65,8 +65,34 @@ struct Tuning
 Tuning g_tune;
 std::mutex g_tune_mu;
 
+// Every entry point works on one snapshot of the knobs, taken when it starts,
+// so that a concurrent pgeof_set_tuning cannot make, e.g., the grid sizing
+// disagree with the workspace carved for it within one call.
+thread_local const Tuning* t_snap = nullptr;
+
+struct TuningSnapshot
+{
+    Tuning v;
+    const Tuning* prev;
+    TuningSnapshot() : prev(t_snap)
+    {
+        {
+            std::lock_guard<std::mutex> l(g_tune_mu);
+            v = g_tune;
+        }
+        if (!prev) t_snap = &v;
+    }
+    ~TuningSnapshot()
+    {
+        if (!prev) t_snap = nullptr;
+    }
+    TuningSnapshot(const TuningSnapshot&) = delete;
+    TuningSnapshot& operator=(const TuningSnapshot&) = delete;
+};
+
 Tuning tuning()
 {
+    if (t_snap) return *t_snap;
     std::lock_guard<std::mutex> l(g_tune_mu);
     return g_tune;
 }
@@ -130,11 +156,20 @@ int ctx_prepare(hipStream_t* s)
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
     if (t_ctx.dev != dev)
     {
+        // this thread moved to another device: release the old device's
+        // stream and buffer on that device (the pinned host buffer is usable
+        // from every device and is kept)
+        if (t_ctx.dev >= 0 && (t_ctx.stream || t_ctx.dbuf) && hipSetDevice(t_ctx.dev) == hipSuccess)
+        {
+            if (t_ctx.stream) (void)hipStreamSynchronize(t_ctx.stream);
+            if (t_ctx.dbuf) (void)hipFree(t_ctx.dbuf);
+            if (t_ctx.stream) (void)hipStreamDestroy(t_ctx.stream);
+            (void)hipSetDevice(dev);
+        }
         t_ctx.dev = dev;
         t_ctx.stream = nullptr;
         t_ctx.dbuf = nullptr;
         t_ctx.dbytes = 0;
-        t_ctx.pinned = nullptr;
     }
     if (!t_ctx.stream) HIP_TRY(hipStreamCreateWithFlags(&t_ctx.stream, hipStreamNonBlocking));
     *s = t_ctx.stream;
@@ -357,7 +392,7 @@ int readback(void* host, const void* dev, size_t bytes, hipStream_t s)
     return PGEOF_OK;
 }
 
-// Sort `data` into a pyramid of uniform grids (cell edge h_0 * 4^l) and,
+// Sort `data` into a pyramid of uniform grids (cell edge h_0 * ratio^l) and,
 // when query != data, sort the queries into level 0.  kNN sizes h_0 from the
 // target occupancy (k used for sizing); radius searches use one level with
 // cells of radius_cell_factor * radius.
@@ -906,6 +941,7 @@ int pgeof_profile_report(char* buf, size_t len)
 int pgeof_knn_search(const float* data, int64_t n_data, const float* query, int64_t n_query, uint32_t knn,
                      uint32_t* indices, float* sqr_dist)
 {
+    TuningSnapshot snapshot;
     int rc = validate_knn(n_data, n_query, knn, false);
     if (rc) return rc;
     if (n_query == 0 || knn == 0) return PGEOF_OK;
@@ -938,6 +974,7 @@ int pgeof_knn_search(const float* data, int64_t n_data, const float* query, int6
 int pgeof_radius_search(const float* data, int64_t n_data, const float* query, int64_t n_query,
                         float search_radius, uint32_t max_knn, int32_t* indices, float* sqr_dist)
 {
+    TuningSnapshot snapshot;
     int rc = validate_knn(n_data, n_query, max_knn, true);
     if (rc) return rc;
     if (n_query == 0 || max_knn == 0) return PGEOF_OK;
@@ -971,6 +1008,7 @@ int pgeof_radius_search(const float* data, int64_t n_data, const float* query, i
 int pgeof_compute_features(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
                            const uint32_t* nn_ptr, int64_t n_nn_ptr, uint64_t k_min, float* features)
 {
+    TuningSnapshot snapshot;
     if (k_min < 1) return fail(PGEOF_EINVAL, "k_min should be > 1");
     int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
     if (rc) return rc;
@@ -982,6 +1020,7 @@ int pgeof_compute_features_multiscale(const float* xyz, int64_t n_xyz, const uin
                                       const uint32_t* nn_ptr, int64_t n_nn_ptr, const uint32_t* k_scales,
                                       int64_t n_scales, float* features)
 {
+    TuningSnapshot snapshot;
     if (n_scales < 0 || !check_scales(k_scales, n_scales))
         return fail(PGEOF_EINVAL, "k_scales should be > 1 and sorted in ascending order");
     int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
@@ -994,6 +1033,7 @@ int pgeof_compute_features_optimal(const float* xyz, int64_t n_xyz, const uint32
                                    const uint32_t* nn_ptr, int64_t n_nn_ptr, uint32_t k_min, uint32_t k_step,
                                    uint32_t k_min_search, float* features)
 {
+    TuningSnapshot snapshot;
     if (k_min < 1 && k_min_search < 1) return fail(PGEOF_EINVAL, "k_min and k_min_search should be > 1");
     if (k_step < 1) return fail(PGEOF_EINVAL, "k_step should be >= 1");
     int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
@@ -1004,29 +1044,34 @@ int pgeof_compute_features_optimal(const float* xyz, int64_t n_xyz, const uint32
 int pgeof_compute_features_selected_f32(const float* xyz, int64_t n, float search_radius, uint32_t max_knn,
                                         const int32_t* selected, int64_t n_selected, float* features)
 {
+    TuningSnapshot snapshot;
     return selected_host<float>(xyz, n, search_radius, max_knn, selected, n_selected, features);
 }
 
 int pgeof_compute_features_selected_f64(const double* xyz, int64_t n, double search_radius, uint32_t max_knn,
                                         const int32_t* selected, int64_t n_selected, double* features)
 {
+    TuningSnapshot snapshot;
     return selected_host<double>(xyz, n, search_radius, max_knn, selected, n_selected, features);
 }
 
 // ---------------------------------------------------- device searches ----
 size_t pgeof_knn_workspace_size(int64_t n_data, int64_t n_query, uint32_t knn)
 {
+    TuningSnapshot snapshot;
     return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), knn, false);
 }
 
 size_t pgeof_radius_workspace_size(int64_t n_data, int64_t n_query, uint32_t max_knn)
 {
+    TuningSnapshot snapshot;
     return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), max_knn,
                            false);
 }
 
 size_t pgeof_radius_csr_workspace_size(int64_t n, uint32_t max_knn)
 {
+    TuningSnapshot snapshot;
     return search_ws_bytes((uint64_t)std::max<int64_t>(n, 0), (uint64_t)std::max<int64_t>(n, 0), max_knn, true);
 }
 
@@ -1034,6 +1079,7 @@ int pgeof_knn_search_device(const float* data, int64_t n_data, const float* quer
                             uint32_t* indices, float* sqr_dist, void* workspace, size_t workspace_bytes,
                             void* stream)
 {
+    TuningSnapshot snapshot;
     int rc = validate_knn(n_data, n_query, knn, false);
     if (rc) return rc;
     if (n_query == 0 || knn == 0) return PGEOF_OK;
@@ -1050,6 +1096,7 @@ int pgeof_radius_search_device(const float* data, int64_t n_data, const float* q
                                float search_radius, uint32_t max_knn, int32_t* indices, float* sqr_dist,
                                void* workspace, size_t workspace_bytes, void* stream)
 {
+    TuningSnapshot snapshot;
     int rc = validate_knn(n_data, n_query, max_knn, true);
     if (rc) return rc;
     if (n_query == 0 || max_knn == 0) return PGEOF_OK;
@@ -1066,6 +1113,7 @@ int pgeof_knn_features_device(const float* xyz, int64_t n, uint32_t knn, uint32_
                               float* sqr_dist, float* features, void* workspace, size_t workspace_bytes,
                               void* stream)
 {
+    TuningSnapshot snapshot;
     int rc = validate_knn(n, n, knn, false);
     if (rc) return rc;
     if (knn < 1) return fail(PGEOF_EINVAL, "knn should be >= 1");
@@ -1087,6 +1135,7 @@ int pgeof_radius_csr_device(const float* xyz, int64_t n, float search_radius, ui
                             uint32_t* nn_ptr, int64_t* nnz_out, void* workspace, size_t workspace_bytes,
                             void* stream)
 {
+    TuningSnapshot snapshot;
     int rc = validate_knn(n, n, max_knn, true);
     if (rc) return rc;
     if ((uint64_t)n * max_knn > 0xFFFFFFFFull) return fail(PGEOF_EINVAL, "n * max_knn exceeds the uint32 CSR range");
@@ -1161,6 +1210,7 @@ int pgeof_compute_features_device(const float* xyz, int64_t n_xyz, const uint32_
                                   const uint32_t* nn_ptr, int64_t n_nn_ptr, uint64_t k_min, float* features,
                                   uint32_t* err, void* stream)
 {
+    TuningSnapshot snapshot;
     if (k_min < 1) return fail(PGEOF_EINVAL, "k_min should be > 1");
     int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
     if (rc) return rc;
@@ -1174,6 +1224,7 @@ int pgeof_compute_features_multiscale_device(const float* xyz, int64_t n_xyz, co
                                              const uint32_t* k_scales_host, int64_t n_scales, float* features,
                                              uint32_t* err, void* stream)
 {
+    TuningSnapshot snapshot;
     if (n_scales < 0 || !check_scales(k_scales_host, n_scales))
         return fail(PGEOF_EINVAL, "k_scales should be > 1 and sorted in ascending order");
     int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
@@ -1188,6 +1239,7 @@ int pgeof_compute_features_optimal_device(const float* xyz, int64_t n_xyz, const
                                           uint32_t k_step, uint32_t k_min_search, float* features, uint32_t* err,
                                           void* stream)
 {
+    TuningSnapshot snapshot;
     if (k_min < 1 && k_min_search < 1) return fail(PGEOF_EINVAL, "k_min and k_min_search should be > 1");
     if (k_step < 1) return fail(PGEOF_EINVAL, "k_step should be >= 1");
     int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
