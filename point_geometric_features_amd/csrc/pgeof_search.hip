@@ -441,6 +441,7 @@ struct Heap4
     uint64_t* keys;    // this lane's pair 0; pair p at keys + p * pstride
     uint32_t pstride;  // entries between consecutive pairs of one lane (2 * block size)
     uint32_t k;
+    uint32_t count;    // keys inserted, up to k
     uint64_t rkey;     // node 0
     uint64_t n1, n2, n3, n4;  // nodes 1..4 (separate scalars, never an indexed array)
 
@@ -529,11 +530,16 @@ struct Heap4
     }
 
     // Start full of the sentinel (larger than or equal to every key that can
-    // be accepted): the heap is valid from the start, so every insertion is
-    // the same replace-the-root sift, wherever each lane is in its scan; no
-    // lane switches from filling to sifting at its own moment.
+    // be accepted), so the heap is valid from the start.  The first k keys
+    // replace the sentinels from the last node upwards, each sifted down from
+    // its own node (a bottom-up heap construction spread over the
+    // insertions: most of them land on leaves and stop at once); the root
+    // stays a sentinel until the k-th key.  Later keys replace the root.
+    // Both cases are one sift from a computed node, with no per-lane switch
+    // between code paths.
     __device__ __forceinline__ void init(uint64_t sentinel)
     {
+        count = 0;
         rkey = n1 = n2 = n3 = n4 = sentinel;
         const ulonglong2 ss = make_ulonglong2(sentinel, sentinel);
         for (uint32_t p = 0; 4 * p < (k > 5 ? 2 * (k - 5) : 0); ++p)
@@ -544,8 +550,10 @@ struct Heap4
     // are in)
     __device__ __forceinline__ void add(uint64_t key, uint64_t& worst)
     {
-        sift_from(0, key, k);
-        worst = rkey;
+        const bool filling = count < k;
+        sift_from(filling ? k - 1 - count : 0u, key, k);
+        count += filling ? 1u : 0u;
+        worst = count >= k ? rkey : worst;
     }
 
     // sort ascending (entries never filled keep the sentinel)
