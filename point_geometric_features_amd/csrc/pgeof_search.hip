@@ -649,7 +649,7 @@ __device__ __forceinline__ Level pick_level(const SearchArgs& a, int l) { return
 constexpr uint32_t kScanUnroll = PGEOF_SCAN_UNROLL;  // candidates loaded together per lane
 
 template <int MODE, int WAVES, bool GLOBAL_HEAP, bool BUDGET>
-__global__ __launch_bounds__(WAVES * 64) void k_search(SearchArgs a)
+__global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
 {
     constexpr bool RADIUS = (MODE == kModeRadius || MODE == kModeRadiusCsr);
     constexpr int BLOCK = WAVES * 64;
