@@ -193,7 +193,9 @@ void pgeof_server_destroy(pgeof_server* server);
  * not found k candidates is deferred; 1..budget_cells), "budget_levels"
  * (pyramid levels, finest first, that get a budgeted pass; coarser levels
  * serve the unbudgeted pass only), "grid_level_ratio" (cell edge
- * ratio between consecutive pyramid levels, 2..8).
+ * ratio between consecutive pyramid levels, 2..8), "csr_order" (1: the
+ * optimal-k kernel visits the rows of inputs of 65536 rows or more in the
+ * spatial order of their first neighbour, 0: in index order).
  * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);
 double pgeof_get_tuning(const char* key);

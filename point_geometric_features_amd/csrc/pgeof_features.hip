@@ -11,6 +11,8 @@
 // Every neighbour index and every nn_ptr entry is bounds-checked; a bad value
 // zeroes the row and raises a bit in the error word instead of reading out of
 // bounds (the reference leaves this undefined, pca.hpp:103-104).
+#include <algorithm>
+
 #include "pgeof_device.hpp"
 #include "pgeof_internal.hpp"
 
@@ -39,8 +41,11 @@ __device__ __forceinline__ void features_of(const Moments& m, uint32_t k, float 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= a.n_points) return;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= a.n_points) return;
+    // rows are visited in the spatial order of their first neighbour when an
+    // order is given, so that a wave's gathers share cache lines
+    const uint32_t i = a.order ? a.order[t] : t;
     const uint32_t b = a.nn_ptr[i], e = a.nn_ptr[i + 1];
     const uint32_t nf = MODE == kCsrOptimal ? 12u : 11u;
     const uint32_t n_out = MODE == kCsrMulti ? a.n_scales * 11u : nf;
@@ -180,7 +185,106 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
     }
 }
 
+// ---- spatial order of CSR rows ----
+// Key of a row: the Morton code (kMortonBits per axis) of its first
+// neighbour in the bounding box of xyz (read from device memory, no host
+// round trip); 0 for an empty or invalid row (the feature kernel reports it).
+constexpr int kMortonBits = 7;  // 2^21 buckets
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    uint32_t r = 0;
+    for (int b = 0; b < kMortonBits; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_row_keys(const float* __restrict__ xyz, uint32_t n_xyz,
+                                                  const uint32_t* __restrict__ nn, uint64_t n_nn,
+                                                  const uint32_t* __restrict__ nn_ptr, uint32_t n_points,
+                                                  const float* __restrict__ bbox, uint32_t* __restrict__ key)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n_points) return;
+    const uint32_t b = nn_ptr[i], e = nn_ptr[i + 1];
+    uint32_t k = 0;
+    if (b < e && (uint64_t)e <= n_nn)
+    {
+        const uint32_t j = nn[b];
+        if (j < n_xyz)
+        {
+            const float side = (float)(1u << kMortonBits);
+            uint32_t c[3];
+            for (int ax = 0; ax < 3; ++ax)
+            {
+                const float lo = bbox[ax], ext = bbox[3 + ax] - bbox[ax];
+                float f = ext > 0.0f ? (xyz[(size_t)j * 3 + ax] - lo) / ext * side : 0.0f;
+                f = fminf(fmaxf(f, 0.0f), side - 1.0f);  // also maps NaN to 0
+                c[ax] = (uint32_t)f;
+            }
+            k = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+        }
+    }
+    key[i] = k;
+}
+
+__global__ __launch_bounds__(256) void k_key_hist(const uint32_t* __restrict__ key, uint32_t n,
+                                                  uint32_t* __restrict__ count, uint32_t* __restrict__ rank)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) rank[i] = atomicAdd(&count[key[i]], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_key_scatter(const uint32_t* __restrict__ key,
+                                                     const uint32_t* __restrict__ rank, uint32_t n,
+                                                     const uint32_t* __restrict__ start, uint32_t* __restrict__ order)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) order[start[key[i]] + rank[i]] = i;
+}
+
 }  // namespace
+
+size_t csr_order_bytes(uint32_t n_points)
+{
+    const uint64_t buckets = 1ull << (3 * kMortonBits);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    return al(1024 * 6 * 4) + al(16 * 4) + 3 * al((size_t)n_points * 4) + al((buckets + 1) * 4) +
+           al(scan_partials_needed(buckets + 1) * 4);
+}
+
+hipError_t launch_csr_order(const CsrArgs& a, void* ws, uint32_t* order_out_ptr_holder[1], hipStream_t s)
+{
+    const uint32_t n = a.n_points;
+    const uint64_t buckets = 1ull << (3 * kMortonBits);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    char* p = (char*)ws;
+    float* bb_part = (float*)p;
+    p += al(1024 * 6 * 4);
+    float* bb = (float*)p;
+    p += al(16 * 4);
+    uint32_t* key = (uint32_t*)p;
+    p += al((size_t)n * 4);
+    uint32_t* rank = (uint32_t*)p;
+    p += al((size_t)n * 4);
+    uint32_t* order = (uint32_t*)p;
+    p += al((size_t)n * 4);
+    uint32_t* count = (uint32_t*)p;
+    p += al((buckets + 1) * 4);
+    uint32_t* part = (uint32_t*)p;
+    ProfScope ps("csr_row_order", s);
+    const uint32_t nbb = std::max<uint32_t>(1, std::min<uint32_t>(1024, (a.n_xyz + 255) / 256));
+    hipError_t e = launch_bbox(a.xyz, a.n_xyz, bb_part, nbb, bb, s);
+    if (e != hipSuccess) return e;
+    k_row_keys<<<(n + 255) / 256, 256, 0, s>>>(a.xyz, a.n_xyz, a.nn, a.n_nn, a.nn_ptr, n, bb, key);
+    e = launch_fill_u32(count, buckets + 1, 0u, s);
+    if (e != hipSuccess) return e;
+    k_key_hist<<<(n + 255) / 256, 256, 0, s>>>(key, n, count, rank);
+    e = launch_scan_exclusive(count, count, buckets + 1, part, s);
+    if (e != hipSuccess) return e;
+    k_key_scatter<<<(n + 255) / 256, 256, 0, s>>>(key, rank, n, count, order);
+    order_out_ptr_holder[0] = order;
+    return hipGetLastError();
+}
 
 hipError_t launch_csr_features(int mode, const CsrArgs& a, hipStream_t s)
 {

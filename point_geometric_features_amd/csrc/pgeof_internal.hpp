@@ -159,8 +159,14 @@ struct CsrArgs
     uint32_t n_scales;
     float* out;
     uint32_t* err;
+    const uint32_t* order;  // rows in this order (a permutation of 0..n_points-1), or nullptr
 };
 hipError_t launch_csr_features(int mode, const CsrArgs& a, hipStream_t s);
+// spatial order of the rows of `a` (Morton code of each row's first neighbour)
+// in a workspace of csr_order_bytes(n_points) bytes; *order receives the
+// permutation (inside the workspace)
+size_t csr_order_bytes(uint32_t n_points);
+hipError_t launch_csr_order(const CsrArgs& a, void* ws, uint32_t* order[1], hipStream_t s);
 
 // ---- profiling hooks (pgeof_capi.cpp) ----
 void prof_begin(const char* name, hipStream_t s, void** token);

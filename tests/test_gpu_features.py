@@ -193,3 +193,32 @@ def test_optimal_errors():
         pgeof.compute_features_optimal(xyz, nn, ptr, 1, 0, 1)
     # k_min = 0 alone is accepted (the reference checks k_min < 1 && k_min_search < 1)
     pgeof.compute_features_optimal(xyz, nn, ptr, 0, 1, 1)
+
+
+def test_csr_spatial_row_order_changes_nothing():
+    """Optimal k on 65536 rows or more visits the rows in spatial order; the
+    results are the same bits as in index order, and bad rows still raise."""
+    from point_geometric_features_amd import capi
+
+    xyz = scenes.lidar_scene(80_000, seed=33, extent=25.0)
+    idx, _ = pgeof.knn_search(xyz, xyz, 24)
+    nn, ptr = scenes.knn_csr(idx)
+    old = capi.get_tuning("csr_order")
+    try:
+        res = {}
+        for order in (1, 0):
+            capi.set_tuning("csr_order", order)
+            res[order] = (pgeof.compute_features(xyz, nn, ptr),
+                          pgeof.compute_features_multiscale(xyz, nn, ptr, [8, 16, 24]),
+                          pgeof.compute_features_optimal(xyz, nn, ptr, 8, 2, 8))
+        capi.set_tuning("csr_order", 1)
+        bad = nn.copy()
+        bad[len(bad) // 2] = len(xyz)
+        with pytest.raises(ValueError):
+            pgeof.compute_features_optimal(xyz, bad, ptr, 8, 2, 8)
+    finally:
+        capi.set_tuning("csr_order", old)
+    for a, b in zip(res[1], res[0]):
+        np.testing.assert_array_equal(a, b)
+    ref = O.compute_features(xyz, nn, ptr)
+    assert_features_close(res[1][0], ref, O.eigvals_csr_np(xyz, nn, ptr), "spatial row order")
