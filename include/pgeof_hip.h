@@ -195,7 +195,9 @@ void pgeof_server_destroy(pgeof_server* server);
  * serve the unbudgeted pass only), "grid_level_ratio" (cell edge
  * ratio between consecutive pyramid levels, 2..8), "csr_order" (1: the
  * optimal-k kernel visits the rows of inputs of 65536 rows or more in the
- * spatial order of their first neighbour, 0: in index order).
+ * spatial order of their first neighbour, 0: in index order), "heap_tail"
+ * (1: search heaps of k > 21 keep nodes 21..k-1 in global memory and only
+ * nodes 5..20 in LDS, 0: the whole heap in LDS while it fits).
  * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);
 double pgeof_get_tuning(const char* key);

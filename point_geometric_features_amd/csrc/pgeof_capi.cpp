@@ -62,6 +62,7 @@ struct Tuning
     double budget_levels = 1.0;         // pyramid levels that get a budgeted pass (the rest: unbudgeted only)
     double grid_level_ratio = 2.0;      // cell edge ratio between consecutive pyramid levels (2..8)
     double csr_order = 1.0;             // CSR features: visit large inputs' rows in spatial order
+    double heap_tail = 1.0;             // search heaps: nodes past 20 in global memory (k > 21)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -304,6 +305,7 @@ struct HeapPlan
     int block;
     size_t lds;
     bool global;
+    uint32_t tail_pairs = 0;  // pairs of heap nodes per lane in global memory (nodes 21..k-1, whole groups of four)
 };
 
 HeapPlan heap_plan_bytes(uint32_t k, size_t entry_bytes)
@@ -319,15 +321,29 @@ HeapPlan heap_plan_bytes(uint32_t k, size_t entry_bytes)
 // Search kernels: per wave 64 heaps plus a 128-point staging buffer (2 KiB)
 // in LDS.  A heap keeps nodes 0..4 in registers and nodes 5..k-1 in groups of
 // four slots: ceil((k - 5) / 4) groups of four 8-byte keys per lane.
+// With the heap tail (k > 21), nodes 5..20 stay in LDS (128 bytes per lane)
+// and nodes 21..k-1 go to global memory as lane-interleaved 16-byte pairs.
 HeapPlan heap_plan(uint32_t k)
 {
     const Tuning t = tuning();
-    const size_t per = (size_t)4 * (k > 5 ? (k - 2) / 4 : 0) * 8;
+    const bool tail = t.heap_tail > 0.5 && k > 21;
+    const uint32_t groups = k > 5 ? (k - 2) / 4 : 0;
+    const size_t per = (size_t)4 * (tail ? 4 : groups) * 8;
     const size_t stage = 128 * 16;
     const size_t budget = (size_t)std::min<double>(t.lds_budget, (double)lds_limit());
     for (int b : {256, 128, 64})
-        if (per * (size_t)b + stage * (size_t)(b / 64) <= budget) return {b, per * (size_t)b + stage * (size_t)(b / 64), false};
-    return {256, 0, true};
+        if (per * (size_t)b + stage * (size_t)(b / 64) <= budget)
+            return {b, per * (size_t)b + stage * (size_t)(b / 64), false, tail ? 2 * ((k - 21 + 3) / 4) : 0u};
+    return {256, 0, true, 0u};
+}
+
+// global-memory heap storage of a search over m queries: the whole heap
+// (fallback) or the tail of every lane of the launch (grid rounded up to 256)
+size_t heap_global_bytes(const HeapPlan& hp, uint64_t m, uint32_t k)
+{
+    if (hp.global) return (size_t)m * k * 8;
+    if (hp.tail_pairs) return (size_t)(m + 256) * hp.tail_pairs * 16;
+    return 0;
 }
 
 size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool csr)
@@ -341,7 +357,7 @@ size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool csr)
     b += 2 * al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4) + 2 * al(m * 16) + 3 * al(m * 4);
     b += al(kMaxLevels * sizeof(Level));
     const HeapPlan hp = heap_plan(k);
-    if (hp.global) b += al(m * k * 8);
+    if (const size_t hb = heap_global_bytes(hp, m, k)) b += al(hb);
     if (csr) b += al(m * k * 4) + al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4);
     return b + 4096;
 }
@@ -372,7 +388,7 @@ SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool csr)
     w.blist2 = ar.take<float>(m);
     w.levels = ar.take<Level>(kMaxLevels);
     const HeapPlan hp = heap_plan(k);
-    if (hp.global) w.gheap_keys = ar.take<uint64_t>(m * k);
+    if (const size_t hb = heap_global_bytes(hp, m, k)) w.gheap_keys = ar.take<uint64_t>(hb / 8);
     if (csr)
     {
         w.dense = ar.take<uint32_t>(m * k);
@@ -542,6 +558,7 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     a.out_count = csr ? w.counts : out_count;
     const HeapPlan hp = heap_plan(k);
     a.gheap_keys = w.gheap_keys;
+    a.heap_tail = hp.tail_pairs ? 1 : 0;
     if (mode == kModeKnn || mode == kModeKnnFeat)
     {
         // Budgeted passes, finest level first: a query whose k-th neighbour
@@ -892,6 +909,8 @@ int pgeof_set_tuning(const char* key, double v)
         g_tune.grid_level_ratio = v;
     else if (k == "csr_order")
         g_tune.csr_order = v;
+    else if (k == "heap_tail")
+        g_tune.heap_tail = v;
     else
         return fail(PGEOF_EINVAL, "unknown tuning key " + k);
     return PGEOF_OK;
@@ -912,6 +931,7 @@ double pgeof_get_tuning(const char* key)
     if (k == "budget_levels") return g_tune.budget_levels;
     if (k == "grid_level_ratio") return g_tune.grid_level_ratio;
     if (k == "csr_order") return g_tune.csr_order;
+    if (k == "heap_tail") return g_tune.heap_tail;
     return -1.0;
 }
 

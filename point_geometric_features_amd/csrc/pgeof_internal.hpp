@@ -62,7 +62,8 @@ struct SearchArgs
     float* out_d2;        // (n_query, k) or nullptr
     float* out_feat;      // (n_query, 11) or nullptr (fused features)
     uint32_t* out_count;  // per original query: number of hits (radius) or nullptr
-    uint64_t* gheap_keys;  // global-memory heap fallback: k * n_query keys
+    uint64_t* gheap_keys;  // global-memory heap fallback: k * n_query keys; or, with heap_tail, heap nodes past 20
+    int heap_tail;         // heap nodes past 20 in gheap_keys: pair p of lane t at (p * lanes + t) * 2
     uint32_t* defer_flag;  // budgeted pass: 1 where the query is left to the next pass
     const float* query_bound;  // per query: upper bound of its k-th squared distance, or +inf (nullable)
     float* defer_bound;        // budgeted pass: that bound for the deferred queries

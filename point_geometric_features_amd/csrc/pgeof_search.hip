@@ -435,20 +435,46 @@ struct Heap
 // of the block is contiguous, 16 bytes per lane, so a wave reads one pair with
 // conflict-free 16-byte loads.  A sift from the root of a 20-entry heap reads
 // four registers and one LDS group; the binary-heap equivalent walks four
-// levels of LDS.
+// levels of LDS.  With TAIL (large k) only nodes 5..20 (groups 0..3, 128
+// bytes per lane) are in LDS and nodes 21..k-1 in global memory, laid out
+// like the LDS pairs with the launch's lanes in place of the block's: LDS
+// then stays small enough for full occupancy whatever k is, and only sifts
+// that reach the fourth level touch global memory.
+template <bool TAIL>
 struct Heap4
 {
     uint64_t* keys;    // this lane's pair 0; pair p at keys + p * pstride
     uint32_t pstride;  // entries between consecutive pairs of one lane (2 * block size)
     uint32_t k;
     uint32_t count;    // keys inserted, up to k
+    uint64_t* gkeys;   // this lane's first global pair; pair p at gkeys + p * gpstride
+    uint32_t gpstride; // entries between consecutive global pairs of one lane (2 * lanes of the launch)
     uint64_t rkey;     // node 0
     uint64_t n1, n2, n3, n4;  // nodes 1..4 (separate scalars, never an indexed array)
 
-    __device__ __forceinline__ uint32_t slot(uint32_t i) const
+    static constexpr uint32_t kLdsTop = 20;  // with a tail: LDS holds nodes 5..20 (groups 0..3)
+
+    __device__ __forceinline__ static bool in_lds(uint32_t i) { return !TAIL || i <= kLdsTop; }
+    __device__ __forceinline__ uint32_t slot(uint32_t i) const  // LDS node
     {
         const uint32_t s = i - 5;
         return (s >> 1) * pstride + (s & 1);
+    }
+    __device__ __forceinline__ uint32_t gslot(uint32_t i) const  // global node
+    {
+        const uint32_t s = i - (kLdsTop + 1);
+        return (s >> 1) * gpstride + (s & 1);
+    }
+    __device__ __forceinline__ uint64_t load(uint32_t i) const  // i >= 5
+    {
+        return in_lds(i) ? keys[slot(i)] : gkeys[gslot(i)];
+    }
+    __device__ __forceinline__ void store(uint32_t i, uint64_t kk)  // i >= 5
+    {
+        if (in_lds(i))
+            keys[slot(i)] = kk;
+        else
+            gkeys[gslot(i)] = kk;
     }
     // The register nodes are combined with bit masks, never indexed or
     // stored in a branch: a select chain over consecutive members would be
@@ -468,21 +494,32 @@ struct Heap4
         n3 = (kk & m3) | (n3 & ~m3);
         n4 = (kk & m4) | (n4 & ~m4);
     }
-    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return i <= 4 ? reg(i) : keys[slot(i)]; }
+    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return i <= 4 ? reg(i) : load(i); }
     __device__ __forceinline__ void put(uint32_t i, uint64_t kk)
     {
         if (i <= 4)
             set_reg(i, kk);
         else
-            keys[slot(i)] = kk;
+            store(i, kk);
     }
 
-    // largest of the children 4i+1..4i+4 (< n) of node i >= 1, which are LDS group i - 1
+    // largest of the children 4i+1..4i+4 (< n) of node i >= 1: group i - 1,
+    // in LDS (or, with a tail, in global memory past group 3)
     __device__ __forceinline__ uint64_t max_child(uint32_t i, uint32_t n, uint32_t& c) const
     {
         const uint32_t c0 = 4 * i + 1, g = i - 1;
-        const ulonglong2 g01 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g) * pstride);
-        const ulonglong2 g23 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g + 1) * pstride);
+        ulonglong2 g01, g23;
+        if (in_lds(c0))
+        {
+            g01 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g) * pstride);
+            g23 = *reinterpret_cast<const ulonglong2*>(keys + (2 * g + 1) * pstride);
+        }
+        else
+        {
+            const uint32_t gg = g - (kLdsTop - 4) / 4;
+            g01 = *reinterpret_cast<const ulonglong2*>(gkeys + (2 * gg) * gpstride);
+            g23 = *reinterpret_cast<const ulonglong2*>(gkeys + (2 * gg + 1) * gpstride);
+        }
         uint64_t mk = g01.x;
         uint32_t mj = 0;
         if (c0 + 1 < n && g01.y > mk) mk = g01.y, mj = 1;
@@ -493,7 +530,7 @@ struct Heap4
     }
 
     // put `key` at node i of the heap of nodes [0, n) and sift it down:
-    // node 0, then one of nodes 1..4 (registers), then LDS nodes only
+    // node 0, then one of nodes 1..4 (registers), then memory nodes only
     __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t n)
     {
         if (i == 0)
@@ -523,10 +560,10 @@ struct Heap4
             uint32_t c;
             const uint64_t mk = max_child(i, n, c);
             if (mk <= key) break;
-            keys[slot(i)] = mk;
+            store(i, mk);
             i = c;
         }
-        keys[slot(i)] = key;
+        store(i, key);
     }
 
     // Start full of the sentinel (larger than or equal to every key that can
@@ -542,8 +579,11 @@ struct Heap4
         count = 0;
         rkey = n1 = n2 = n3 = n4 = sentinel;
         const ulonglong2 ss = make_ulonglong2(sentinel, sentinel);
-        for (uint32_t p = 0; 4 * p < (k > 5 ? 2 * (k - 5) : 0); ++p)
-            *reinterpret_cast<ulonglong2*>(keys + p * pstride) = ss;
+        const uint32_t pairs = k > 5 ? (k - 5 + 1) / 2 : 0;  // pairs of nodes 5..k-1
+        const uint32_t lds_pairs = TAIL ? min(pairs, (kLdsTop - 5 + 1) / 2) : pairs;
+        for (uint32_t p = 0; p < lds_pairs; ++p) *reinterpret_cast<ulonglong2*>(keys + p * pstride) = ss;
+        for (uint32_t p = lds_pairs; p < pairs; ++p)
+            *reinterpret_cast<ulonglong2*>(gkeys + (p - lds_pairs) * gpstride) = ss;
     }
 
     // precondition: key < worst, worst = the root (the sentinel until k keys
@@ -562,8 +602,8 @@ struct Heap4
         uint32_t end = k - 1;
         for (; end > 4; --end)
         {
-            const uint64_t last = keys[slot(end)];
-            keys[slot(end)] = rkey;
+            const uint64_t last = load(end);
+            store(end, rkey);
             sift_from(0, last, end);
         }
         for (; end > 0; --end)
@@ -648,7 +688,12 @@ __device__ __forceinline__ Level pick_level(const SearchArgs& a, int l) { return
 #endif
 constexpr uint32_t kScanUnroll = PGEOF_SCAN_UNROLL;  // candidates loaded together per lane
 
-template <int MODE, int WAVES, bool GLOBAL_HEAP, bool BUDGET>
+// where a search kernel keeps its heaps
+constexpr int kHeapLds = 0;     // 4-ary, registers + LDS
+constexpr int kHeapGlobal = 1;  // binary, global memory (fallback)
+constexpr int kHeapTail = 2;    // 4-ary, registers + LDS + global tail past node 20
+
+template <int MODE, int WAVES, int HEAP, bool BUDGET>
 __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
 {
     constexpr bool RADIUS = (MODE == kModeRadius || MODE == kModeRadiusCsr);
@@ -663,7 +708,8 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
 
     // heaps: 4-ary with nodes 0..4 in registers and ceil((k - 5) / 4) groups
     // of four keys per lane in LDS; global-memory fallback for large k: binary
-    using HeapT = typename std::conditional<GLOBAL_HEAP, Heap, Heap4>::type;
+    constexpr bool GLOBAL_HEAP = HEAP == kHeapGlobal;
+    using HeapT = typename std::conditional<GLOBAL_HEAP, Heap, Heap4<HEAP == kHeapTail>>::type;
     HeapT heap;
     if constexpr (GLOBAL_HEAP)
     {
@@ -675,6 +721,12 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
     {
         heap.keys = reinterpret_cast<uint64_t*>(hbase) + (size_t)threadIdx.x * 2;
         heap.pstride = 2 * BLOCK;
+        if constexpr (HEAP == kHeapTail)
+        {
+            // nodes past 20 in global memory (lane-interleaved pairs)
+            heap.gkeys = a.gheap_keys + (size_t)t * 2;
+            heap.gpstride = 2u * gridDim.x * (uint32_t)BLOCK;
+        }
     }
     heap.k = a.k;
 
@@ -1397,10 +1449,13 @@ hipError_t launch_search_mb(const SearchArgs& a, size_t lds, bool gh, hipStream_
     constexpr int BLOCK = WAVES * 64;
     const uint32_t nb = (a.n_query + BLOCK - 1) / BLOCK;
     if (nb == 0) return hipSuccess;
+    if (a.heap_tail && (gh || !a.gheap_keys)) return hipErrorInvalidValue;
     if (gh)
-        k_search<MODE, WAVES, true, BUDGET><<<nb, BLOCK, (size_t)WAVES * kStage * sizeof(float4), s>>>(a);
+        k_search<MODE, WAVES, kHeapGlobal, BUDGET><<<nb, BLOCK, (size_t)WAVES * kStage * sizeof(float4), s>>>(a);
+    else if (a.heap_tail)
+        k_search<MODE, WAVES, kHeapTail, BUDGET><<<nb, BLOCK, lds, s>>>(a);
     else
-        k_search<MODE, WAVES, false, BUDGET><<<nb, BLOCK, lds, s>>>(a);
+        k_search<MODE, WAVES, kHeapLds, BUDGET><<<nb, BLOCK, lds, s>>>(a);
     return hipGetLastError();
 }
 

@@ -43,7 +43,7 @@ def test_knn_uniform_10k_k20_bitexact():
     assert (np.diff(gd, axis=1) >= 0).all()
 
 
-@pytest.mark.parametrize("k", [1, 2, 4, 5, 6, 7, 20, 21, 33, 64, 100])
+@pytest.mark.parametrize("k", [1, 2, 4, 5, 6, 7, 20, 21, 22, 26, 33, 64, 100])
 def test_knn_various_k(k):
     xyz = scenes.uniform_cube(3000, seed=k)
     _eq_knn(xyz, xyz, k, kdtree=False)
@@ -219,6 +219,7 @@ _KNOB_SETS = [
     {"knn_occupancy": 2.0},
     {"lds_budget": 16384},
     {"lds_budget": 4096},  # heaps in global memory
+    {"heap_tail": 0},  # radius cap 24: the whole heap in LDS instead of nodes 21..23 in global memory
 ]
 
 
