@@ -197,7 +197,10 @@ void pgeof_server_destroy(pgeof_server* server);
  * optimal-k kernel visits the rows of inputs of 65536 rows or more in the
  * spatial order of their first neighbour, 0: in index order), "heap_tail"
  * (1: search heaps of k > 21 keep nodes 21..k-1 in global memory and only
- * nodes 5..20 in LDS, 0: the whole heap in LDS while it fits).
+ * nodes 5..20 in LDS, 0: the whole heap in LDS while it fits), "staging"
+ * (wave-shared LDS staging of cell rows: 0 never, 1 radius search and,
+ * for k >= 32, the budgeted kNN pass, 2 radius search and the budgeted kNN
+ * pass at any k).
  * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);
 double pgeof_get_tuning(const char* key);

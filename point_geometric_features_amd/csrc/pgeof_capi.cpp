@@ -63,6 +63,7 @@ struct Tuning
     double grid_level_ratio = 2.0;      // cell edge ratio between consecutive pyramid levels (2..8)
     double csr_order = 1.0;             // CSR features: visit large inputs' rows in spatial order
     double heap_tail = 1.0;             // search heaps: nodes past 20 in global memory (k > 21)
+    double staging = 1.0;               // LDS staging of cell rows: 0 never, 1 radius + budgeted kNN at k >= 32, 2 always
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -550,6 +551,9 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     a.margin = 1e-6f;
     a.r2 = radius * radius;
     a.near_first = tuning().near_first > 0.5 ? 1 : 0;
+    // staged rows pay off in the budgeted kNN pass only for large k (measured)
+    const int staging = std::max(0, std::min(2, (int)tuning().staging));
+    a.staging = staging == 1 && k >= 32 ? 2 : staging;
     a.budget_cells = std::max(1, std::min(64, (int)tuning().budget_cells));
     a.budget_shells = std::max(1, std::min(a.budget_cells, (int)tuning().budget_shells));
     a.out_idx = csr ? (void*)w.dense : out_idx;
@@ -911,6 +915,8 @@ int pgeof_set_tuning(const char* key, double v)
         g_tune.csr_order = v;
     else if (k == "heap_tail")
         g_tune.heap_tail = v;
+    else if (k == "staging")
+        g_tune.staging = v;
     else
         return fail(PGEOF_EINVAL, "unknown tuning key " + k);
     return PGEOF_OK;
@@ -932,6 +938,7 @@ double pgeof_get_tuning(const char* key)
     if (k == "grid_level_ratio") return g_tune.grid_level_ratio;
     if (k == "csr_order") return g_tune.csr_order;
     if (k == "heap_tail") return g_tune.heap_tail;
+    if (k == "staging") return g_tune.staging;
     return -1.0;
 }
 

@@ -68,6 +68,7 @@ struct SearchArgs
     const float* query_bound;  // per query: upper bound of its k-th squared distance, or +inf (nullable)
     float* defer_bound;        // budgeted pass: that bound for the deferred queries
     int near_first;        // visit cell rows nearest-first
+    int staging;           // wave-shared LDS staging of cell rows: 0 never, 1 radius search, 2 also the budgeted kNN pass
     int budget_cells;      // budgeted pass: largest ball half-width, in cells around the query's cell
     int budget_shells;     // budgeted pass: cubic shells scanned before a query without k candidates is deferred
 };

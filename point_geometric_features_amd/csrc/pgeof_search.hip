@@ -678,6 +678,21 @@ __device__ __forceinline__ int near_index(int i, int c, int lo, int hi)
     return (i & 1) ? c - d : c + d;
 }
 
+// t-th element (t in [0, hi - lo]) of the order c, c-1, c+1, c-2, c+2, ...
+// over [lo, hi] with c in [lo, hi]: alternating while both sides last, then
+// the longer side outwards
+__device__ __forceinline__ int nth_near(int t, int c, int lo, int hi)
+{
+    const int below = c - lo, above = hi - c, m = min(below, above);
+    if (t <= 2 * m)
+    {
+        const int d = (t + 1) >> 1;
+        return (t & 1) ? c - d : c + d;
+    }
+    const int r = t - m;
+    return below > above ? c - r : c + r;
+}
+
 // Level l of the pyramid.  A level that differs between lanes is read from
 // the device copy of the table: indexing the kernel-argument copy with a
 // per-lane value would move the whole argument block to scratch memory.
@@ -879,8 +894,11 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
         const long union_rows = (long)(Z1 - Z0 + 1) * (long)(Y1 - Y0 + 1);
         // The unbudgeted kNN pass always scans per lane: its queries are
         // sparse and far apart, so shared staging costs more than it saves
-        // (measured).
-        const bool coop = (RADIUS || BUDGET) && (lmin == lmax) && union_rows <= 2L * max_rows + 8;
+        // (measured).  The budgeted pass stages only when asked to: its lanes
+        // scan each other's rows in lockstep either way, and per-lane loads
+        // from L2 measure faster than staging the rows' union.
+        const bool stage_here = RADIUS ? a.staging >= 1 : (BUDGET && a.staging >= 2);
+        const bool coop = stage_here && (lmin == lmax) && union_rows <= 2L * max_rows + 8;
         if (coop)
         {
             // every lane stages from the working level's array, also lanes
@@ -938,8 +956,10 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
                 }
             }
         }
-        else if (has)
+        else if (has && BUDGET)
         {
+            // budgeted pass: the lanes walk their rows in step (nearest
+            // first), a row's setup shared by the whole wave
             const int zc = a.near_first ? cz : B.z0, yc = a.near_first ? cy : B.y0;
             for (int iz = 0; iz < 2 * (B.z1 - B.z0 + 1); ++iz)
             {
@@ -971,6 +991,89 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
                         e1 = cs[base + (uint32_t)b1 + 1];
                     }
                     scan_direct(s0, e0, s1, e1);
+                }
+            }
+        }
+        else if (has)
+        {
+            // Unbudgeted pass: each lane walks its own rows (nearest first)
+            // with a cursor and its candidates four per step.  A lane that
+            // finishes a row opens its next one within the same step, so the
+            // lanes of a wave do not wait for each other at row ends: the wave
+            // takes as many steps as its busiest lane, not the sum over rows
+            // of the busiest lane per row.  (In the budgeted pass the rows
+            // are short and alike, and sharing each row's setup wins.)
+            const int zc = a.near_first ? min(max(cz, B.z0), B.z1) : B.z0;
+            const int yc = a.near_first ? min(max(cy, B.y0), B.y1) : B.y0;
+            const int nz = B.z1 - B.z0 + 1, ny = B.y1 - B.y0 + 1;
+            int iz = 0, iy = 0;  // next row to open: row iy (near order) of plane iz
+            int z = nth_near(0, zc, B.z0, B.z1);
+            double dz2 = 0.0;
+            if (tighten)
+            {
+                const double dz = axis_gap(qz, z, G.lo[2], h, margin);
+                dz2 = dz * dz;
+            }
+            bool rows = true;  // rows left to open
+            uint32_t pos = 0, end = 0, ps = 0, pe = 0;  // current piece, pending second piece
+            while (__ballot(rows || pos < end || ps < pe) != 0)
+            {
+                if (pos >= end)
+                {
+                    if (ps < pe)
+                    {
+                        pos = ps;
+                        end = pe;
+                        ps = pe = 0;
+                    }
+                    else
+                    {
+                        // open the next row with a non-empty x range; every
+                        // pass moves the cursor by one row or one plane
+                        while (rows)
+                        {
+                            if (iy >= ny || (tighten && dz2 > wlim()))
+                            {
+                                iy = 0;
+                                if (++iz >= nz)
+                                {
+                                    rows = false;
+                                    break;
+                                }
+                                z = nth_near(iz, zc, B.z0, B.z1);
+                                if (tighten)
+                                {
+                                    const double dz = axis_gap(qz, z, G.lo[2], h, margin);
+                                    dz2 = dz * dz;
+                                }
+                                continue;
+                            }
+                            const int y = nth_near(iy++, yc, B.y0, B.y1);
+                            const int4 rp = row_pieces(B, C, hasC, tighten, dz2, y, z);
+                            const bool p0 = rp.x <= rp.y, p1 = rp.z <= rp.w;
+                            if (!p0 && !p1) continue;
+                            const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
+                            const uint32_t a0 = (uint32_t)(p0 ? rp.x : rp.z), b0 = (uint32_t)(p0 ? rp.y : rp.w);
+                            pos = cs[base + a0];
+                            end = cs[base + b0 + 1];
+                            if (p0 && p1)
+                            {
+                                ps = cs[base + (uint32_t)rp.z];
+                                pe = cs[base + (uint32_t)rp.w + 1];
+                            }
+                            break;
+                        }
+                    }
+                }
+                if (pos < end)
+                {
+                    float4 p[kScanUnroll];
+#pragma unroll
+                    for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = pts[min(pos + u, end - 1)];
+#pragma unroll
+                    for (uint32_t u = 0; u < kScanUnroll; ++u)
+                        if (pos + u < end) consider(p[u], pos + u);
+                    pos = min(pos + kScanUnroll, end);
                 }
             }
         }

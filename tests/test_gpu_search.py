@@ -220,6 +220,8 @@ _KNOB_SETS = [
     {"lds_budget": 16384},
     {"lds_budget": 4096},  # heaps in global memory
     {"heap_tail": 0},  # radius cap 24: the whole heap in LDS instead of nodes 21..23 in global memory
+    {"staging": 0},
+    {"staging": 2},
 ]
 
 
