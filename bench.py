@@ -372,7 +372,8 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": kernel, "kernel_ms": round(avg_kernel_ms, 4),
                     "algorithmic_bytes_per_point": bytes_per_point}
-    extra["kernels_ms_per_step"] = {kk: round(v[1] / max(v[0], 1), 4) for kk, v in sorted(prof.items())}
+    extra["kernels_ms_per_step"] = {kk: round(v[1] / args.steps, 4) for kk, v in sorted(prof.items())}
+    extra["kernel_launches_per_step"] = {kk: round(v[0] / args.steps, 2) for kk, v in sorted(prof.items())}
     lat_ms = np.array(lat) * 1e3
     extra["latency_ms"] = {"p50": round(float(np.percentile(lat_ms, 50)), 3),
                            "p99": round(float(np.percentile(lat_ms, 99)), 3)}
