@@ -51,7 +51,7 @@ int hip_fail(hipError_t e, const char* where)
 // ------------------------------------------------------------- tuning ----
 struct Tuning
 {
-    double knn_occupancy = 0.35;       // target points per cell = max(0.75, this * k)
+    double knn_occupancy = 0.25;       // target points per cell = max(0.75, this * k)
     double grid_cells_per_point = 8.0;  // cap on G / n
     double lds_budget = 65536.0;        // bytes of LDS per workgroup for heaps
     double radius_cell_factor = 0.5;    // cell edge = this * radius
