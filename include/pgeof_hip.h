@@ -182,7 +182,7 @@ void pgeof_server_destroy(pgeof_server* server);
 /* ------------------------------------------------------------------------ */
 
 /* Set a tuning knob: "knn_occupancy" (target points per grid cell, as a
- * fraction of k), "grid_cells_per_point" (cap on grid cells / points),
+ * fraction of k), "knn_cell_points_max" (cap of that target), "grid_cells_per_point" (cap on grid cells / points),
  * "lds_budget" (bytes of LDS per workgroup for the search heaps),
  * "radius_cell_factor" (grid cell edge / search radius), "grid_levels"
  * (depth of the kNN grid pyramid, 1..4), "near_first" (1: cell rows are

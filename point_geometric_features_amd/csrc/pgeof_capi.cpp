@@ -51,7 +51,8 @@ int hip_fail(hipError_t e, const char* where)
 // ------------------------------------------------------------- tuning ----
 struct Tuning
 {
-    double knn_occupancy = 0.25;       // target points per cell = max(0.75, this * k)
+    double knn_occupancy = 0.35;       // target points per cell = max(0.75, min(this * k, knn_cell_points_max))
+    double knn_cell_points_max = 25.0;  // cap of that target (large k)
     double grid_cells_per_point = 8.0;  // cap on G / n
     double lds_budget = 65536.0;        // bytes of LDS per workgroup for heaps
     double radius_cell_factor = 0.5;    // cell edge = this * radius
@@ -447,7 +448,7 @@ int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, ui
         // target occupancy T; first guess assumes a uniformly filled box,
         // corrected by the measured point-weighted occupancy assuming the
         // points lie on 2-D surfaces (LiDAR); clamped by the grid-size cap.
-        const double T = std::max(0.75, t.knn_occupancy * (double)k);
+        const double T = std::max(0.75, std::min(t.knn_occupancy * (double)k, t.knn_cell_points_max));
         double vol = 1.0;
         for (int a = 0; a < 3; ++a) vol *= std::max(ext[a], emax * 1e-3);
         double h1 = std::cbrt(vol * T / std::max<double>(n, 1.0));
@@ -893,6 +894,8 @@ int pgeof_set_tuning(const char* key, double v)
     if (!(v >= 0.0) || !std::isfinite(v)) return fail(PGEOF_EINVAL, "tuning value must be non-negative");
     if (k == "knn_occupancy")
         g_tune.knn_occupancy = v;
+    else if (k == "knn_cell_points_max")
+        g_tune.knn_cell_points_max = v;
     else if (k == "grid_cells_per_point")
         g_tune.grid_cells_per_point = v;
     else if (k == "lds_budget")
@@ -927,6 +930,7 @@ double pgeof_get_tuning(const char* key)
     std::lock_guard<std::mutex> l(g_tune_mu);
     const std::string k = key ? key : "";
     if (k == "knn_occupancy") return g_tune.knn_occupancy;
+    if (k == "knn_cell_points_max") return g_tune.knn_cell_points_max;
     if (k == "grid_cells_per_point") return g_tune.grid_cells_per_point;
     if (k == "lds_budget") return g_tune.lds_budget;
     if (k == "radius_cell_factor") return g_tune.radius_cell_factor;

@@ -217,6 +217,7 @@ _KNOB_SETS = [
     {"grid_cells_per_point": 1, "grid_levels": 1},
     {"grid_cells_per_point": 32, "knn_occupancy": 0.05},
     {"knn_occupancy": 2.0},
+    {"knn_cell_points_max": 2.0},
     {"lds_budget": 16384},
     {"lds_budget": 4096},  # heaps in global memory
     {"heap_tail": 0},  # radius cap 24: the whole heap in LDS instead of nodes 21..23 in global memory
