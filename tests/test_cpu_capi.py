@@ -90,3 +90,37 @@ def test_tuning_roundtrip():
     capi.set_tuning("knn_occupancy", old)
     with pytest.raises(ValueError):
         capi.set_tuning("nope", 1.0)
+
+
+# every knob the header documents, with its default
+_KNOBS = {
+    "knn_occupancy": 0.35, "knn_cell_points_max": 25.0, "grid_cells_per_point": 8.0, "lds_budget": 65536.0,
+    "radius_cell_factor": 0.5, "grid_levels": 3.0, "near_first": 1.0, "budget_cells": 2.0, "budget_shells": 1.0,
+    "budget_levels": 1.0, "grid_level_ratio": 2.0, "csr_order": 1.0, "heap_tail": 1.0, "staging": 1.0,
+}
+
+
+@pytest.mark.parametrize("key", sorted(_KNOBS))
+def test_tuning_keys_and_defaults(key):
+    assert capi.get_tuning(key) == _KNOBS[key]
+    capi.set_tuning(key, 3.0)
+    try:
+        assert capi.get_tuning(key) == 3.0
+    finally:
+        capi.set_tuning(key, _KNOBS[key])
+    with pytest.raises(ValueError):
+        capi.set_tuning(key, -1.0)
+
+
+def test_heap_tail_workspace():
+    """For k > 21 the global heap nodes are part of the workspace bound; the
+    whole-LDS layout (heap_tail 0) needs none for k that fits LDS."""
+    k = 64
+    with_tail = capi.knn_workspace_size(100_000, 100_000, k)
+    capi.set_tuning("heap_tail", 0)
+    try:
+        without = capi.knn_workspace_size(100_000, 100_000, k)
+    finally:
+        capi.set_tuning("heap_tail", 1)
+    pairs = 2 * ((k - 21 + 3) // 4)
+    assert with_tail - without >= (100_000 + 256) * pairs * 16
