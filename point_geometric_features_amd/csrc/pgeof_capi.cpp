@@ -679,15 +679,15 @@ int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, in
     a.out = out;
     a.err = err;
     a.order = nullptr;
-    // Optimal k on large inputs: visit the rows in the spatial order of their
-    // first neighbour (a Morton-code counting sort, stream-ordered scratch),
-    // so the coordinate gathers of a wave hit the same cache lines.  Measured:
-    // it pays for the optimal-k kernel (long rows, many gathers per row:
-    // 2.7 -> 1.8 ms at 1 M rows of 100), not for the plain and multiscale
-    // kernels, whose row reads and output writes then become scattered
-    // (10 M rows of 20: 5.6 -> 8.6 ms).
+    // Plain and optimal-k features on large inputs: visit the rows in the
+    // spatial order of their first neighbour (a Morton-code counting sort,
+    // stream-ordered scratch), so the coordinate gathers of a wave hit the
+    // same cache lines.  Measured: optimal k, 1 M rows of 100: 2.7 -> 1.8 ms;
+    // plain, 10 M rows of 20: 3.9 -> 2.6 ms plus 0.8 ms for the order.  The
+    // multiscale kernel gains what the order costs (2 M rows: 2.0 -> 1.8 +
+    // 0.2 ms) and keeps index order.
     void* tmp = nullptr;
-    if (mode == 2 && a.n_points >= 65536 && n_xyz > 0 && tuning().csr_order > 0.5)
+    if (mode != 1 && a.n_points >= 65536 && n_xyz > 0 && tuning().csr_order > 0.5)
     {
         HIP_TRY(hipMallocAsync(&tmp, csr_order_bytes(a.n_points), s));
         uint32_t* ord[1] = {nullptr};

@@ -196,8 +196,9 @@ def test_optimal_errors():
 
 
 def test_csr_spatial_row_order_changes_nothing():
-    """Optimal k on 65536 rows or more visits the rows in spatial order; the
-    results are the same bits as in index order, and bad rows still raise."""
+    """Plain and optimal-k features on 65536 rows or more visit the rows in
+    spatial order; the results are the same bits as in index order, and bad
+    rows still raise."""
     from point_geometric_features_amd import capi
 
     xyz = scenes.lidar_scene(80_000, seed=33, extent=25.0)
@@ -216,6 +217,8 @@ def test_csr_spatial_row_order_changes_nothing():
         bad[len(bad) // 2] = len(xyz)
         with pytest.raises(ValueError):
             pgeof.compute_features_optimal(xyz, bad, ptr, 8, 2, 8)
+        with pytest.raises(ValueError):
+            pgeof.compute_features(xyz, bad, ptr)
     finally:
         capi.set_tuning("csr_order", old)
     for a, b in zip(res[1], res[0]):
