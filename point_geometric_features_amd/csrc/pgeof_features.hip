@@ -221,7 +221,8 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v)
 __global__ __launch_bounds__(256) void k_row_keys(const float* __restrict__ xyz, uint32_t n_xyz,
                                                   const uint32_t* __restrict__ nn, uint64_t n_nn,
                                                   const uint32_t* __restrict__ nn_ptr, uint32_t n_points,
-                                                  const float* __restrict__ bbox, uint32_t* __restrict__ key)
+                                                  const float* __restrict__ bbox, uint32_t* __restrict__ key,
+                                                  uint32_t* __restrict__ count, uint32_t* __restrict__ rank)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n_points) return;
@@ -245,13 +246,7 @@ __global__ __launch_bounds__(256) void k_row_keys(const float* __restrict__ xyz,
         }
     }
     key[i] = k;
-}
-
-__global__ __launch_bounds__(256) void k_key_hist(const uint32_t* __restrict__ key, uint32_t n,
-                                                  uint32_t* __restrict__ count, uint32_t* __restrict__ rank)
-{
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n) rank[i] = atomicAdd(&count[key[i]], 1u);
+    rank[i] = atomicAdd(&count[k], 1u);  // histogram of the keys, rank within the bucket
 }
 
 __global__ __launch_bounds__(256) void k_key_scatter(const uint32_t* __restrict__ key,
@@ -295,10 +290,9 @@ hipError_t launch_csr_order(const CsrArgs& a, void* ws, uint32_t* order_out_ptr_
     const uint32_t nbb = std::max<uint32_t>(1, std::min<uint32_t>(1024, (a.n_xyz + 255) / 256));
     hipError_t e = launch_bbox(a.xyz, a.n_xyz, bb_part, nbb, bb, s);
     if (e != hipSuccess) return e;
-    k_row_keys<<<(n + 255) / 256, 256, 0, s>>>(a.xyz, a.n_xyz, a.nn, a.n_nn, a.nn_ptr, n, bb, key);
     e = launch_fill_u32(count, buckets + 1, 0u, s);
     if (e != hipSuccess) return e;
-    k_key_hist<<<(n + 255) / 256, 256, 0, s>>>(key, n, count, rank);
+    k_row_keys<<<(n + 255) / 256, 256, 0, s>>>(a.xyz, a.n_xyz, a.nn, a.n_nn, a.nn_ptr, n, bb, key, count, rank);
     e = launch_scan_exclusive(count, count, buckets + 1, part, s);
     if (e != hipSuccess) return e;
     k_key_scatter<<<(n + 255) / 256, 256, 0, s>>>(key, rank, n, count, order);
