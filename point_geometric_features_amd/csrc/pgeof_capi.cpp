@@ -680,7 +680,7 @@ int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, in
     a.err = err;
     a.order = nullptr;
     // Plain and optimal-k features on large inputs: visit the rows in the
-    // spatial order of their first neighbour (a Morton-code counting sort,
+    // spatial order of their points (a Morton-code counting sort,
     // stream-ordered scratch), so the coordinate gathers of a wave hit the
     // same cache lines.  Measured: optimal k, 1 M rows of 100: 2.7 -> 1.8 ms;
     // plain, 10 M rows of 20: 3.9 -> 2.6 ms plus 0.8 ms for the order.  The

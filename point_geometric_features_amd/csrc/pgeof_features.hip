@@ -206,9 +206,12 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
 }
 
 // ---- spatial order of CSR rows ----
-// Key of a row: the Morton code (kMortonBits per axis) of its first
-// neighbour in the bounding box of xyz (read from device memory, no host
-// round trip); 0 for an empty or invalid row (the feature kernel reports it).
+// Key of a row: the Morton code (kMortonBits per axis), in the bounding box
+// of xyz (read from device memory, no host round trip), of the row's own
+// point (row i of a pgeof CSR is point i, whose first neighbour in a search
+// of the cloud against itself is the point itself), read as a stream; a row
+// past the end of xyz takes its first neighbour instead, and an empty or
+// invalid one 0 (the feature kernel reports it).
 constexpr int kMortonBits = 7;  // 2^21 buckets
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v)
@@ -226,11 +229,14 @@ __global__ __launch_bounds__(256) void k_row_keys(const float* __restrict__ xyz,
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n_points) return;
-    const uint32_t b = nn_ptr[i], e = nn_ptr[i + 1];
-    uint32_t k = 0;
-    if (b < e && (uint64_t)e <= n_nn)
+    uint32_t j = i;
+    if (i >= n_xyz)
     {
-        const uint32_t j = nn[b];
+        const uint32_t b = nn_ptr[i], e = nn_ptr[i + 1];
+        j = (b < e && (uint64_t)e <= n_nn) ? nn[b] : n_xyz;
+    }
+    uint32_t k = 0;
+    {
         if (j < n_xyz)
         {
             const float side = (float)(1u << kMortonBits);
