@@ -195,7 +195,7 @@ void pgeof_server_destroy(pgeof_server* server);
  * serve the unbudgeted pass only), "grid_level_ratio" (cell edge
  * ratio between consecutive pyramid levels, 2..8), "csr_order" (1: the
  * plain and optimal-k kernels visit the rows of inputs of 65536 rows or more
- * in the spatial order of their first neighbour, 0: in index order), "heap_tail"
+ * in the spatial order of their points, 0: in index order), "heap_tail"
  * (1: search heaps of k > 21 keep nodes 21..k-1 in global memory and only
  * nodes 5..20 in LDS, 0: the whole heap in LDS while it fits), "staging"
  * (wave-shared LDS staging of cell rows: 0 never, 1 radius search and,

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
 {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= a.n_points) return;
-    // rows are visited in the spatial order of their first neighbour when an
+    // rows are visited in the spatial order of their points when an
     // order is given, so that a wave's gathers share cache lines
     const uint32_t i = a.order ? a.order[t] : t;
     const uint32_t b = a.nn_ptr[i], e = a.nn_ptr[i + 1];
