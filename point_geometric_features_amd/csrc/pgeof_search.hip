@@ -627,13 +627,17 @@ __device__ __forceinline__ double axis_gap(double v, int c, double lo, double h,
 // One thread per query, queries in cell-sorted order.  Each query first
 // picks the finest grid level whose 3x3x3 cell cube around it already holds
 // k points (kNN; radius searches use level 0), so its row count stays small
-// whatever the local density.  Rows are then scanned in one of two ways,
+// whatever the local density.  Rows are then scanned in one of three ways,
 // chosen per wavefront and pass:
-//   - cooperative: when all lanes of the wave use the same level and the
-//     union of their rows is compact, the wave walks that union in lockstep,
-//     stages each row's point slice into LDS with coalesced loads and every
-//     lane scans its own sub-range from LDS;
-//   - per lane: otherwise each lane walks its own rows.
+//   - cooperative (radius searches; budgeted kNN pass for k >= 32): when all
+//     lanes of the wave use the same level and the union of their rows is
+//     compact, the wave walks that union in lockstep, stages each row's
+//     point slice into LDS with coalesced loads and every lane scans its own
+//     sub-range from LDS;
+//   - per lane in step (budgeted kNN pass): each lane walks its own rows,
+//     the wave opening row r of every lane together;
+//   - per lane with a cursor (unbudgeted pass): each lane opens its next row
+//     as soon as its current one ends.
 constexpr int kStage = 128;  // float4 points staged per wave per chunk
 
 // Wavefront reductions over the active lanes (ROCm device library, DPP based).
