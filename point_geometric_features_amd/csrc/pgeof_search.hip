@@ -338,11 +338,15 @@ __global__ __launch_bounds__(256) void k_compact_rows(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ row_ptr,
                                                       uint32_t* __restrict__ nn)
 {
-    const uint64_t total = (uint64_t)n_rows * cap;
-    for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256)
+    // one wave per row, lanes along the row: only the filled entries are
+    // read, both sides coalesced, and no 64-bit division per entry
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, n_waves = gridDim.x * 4u;
+    for (uint32_t r = wave; r < n_rows; r += n_waves)
     {
-        const uint32_t r = (uint32_t)(e / cap), j = (uint32_t)(e % cap);
-        if (j < counts[r]) nn[row_ptr[r] + j] = dense[e];
+        const uint32_t c = min(counts[r], cap), base = row_ptr[r];
+        const uint32_t* __restrict__ src = dense + (size_t)r * cap;
+        for (uint32_t j = lane; j < c; j += 64u) nn[base + j] = src[j];
     }
 }
 
@@ -1852,8 +1856,8 @@ hipError_t launch_compact_rows(const uint32_t* dense, uint32_t n_rows, uint32_t 
 {
     if ((uint64_t)n_rows * cap == 0) return hipSuccess;
     ProfScope ps("radius_compact", s);
-    k_compact_rows<<<blocks_for((uint64_t)n_rows * cap, 256, 65536), 256, 0, s>>>(dense, n_rows, cap, counts,
-                                                                                 row_ptr, nn);
+    k_compact_rows<<<blocks_for((uint64_t)n_rows * 64, 256, 65536), 256, 0, s>>>(dense, n_rows, cap, counts,
+                                                                                row_ptr, nn);
     return hipGetLastError();
 }
 
