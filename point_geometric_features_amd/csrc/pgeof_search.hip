@@ -570,24 +570,22 @@ struct Heap4
         store(i, key);
     }
 
-    // Start full of the sentinel (larger than or equal to every key that can
-    // be accepted), so the heap is valid from the start.  The first k keys
-    // replace the sentinels from the last node upwards, each sifted down from
-    // its own node (a bottom-up heap construction spread over the
+    // The heap starts full of the sentinel (larger than or equal to every
+    // key that can be accepted), so it is valid from the start.  The first k
+    // keys replace the sentinels from the last node upwards, each sifted down
+    // from its own node (a bottom-up heap construction spread over the
     // insertions: most of them land on leaves and stop at once); the root
     // stays a sentinel until the k-th key.  Later keys replace the root.
     // Both cases are one sift from a computed node, with no per-lane switch
-    // between code paths.
+    // between code paths.  Only the register nodes are set here: a sift from
+    // node s reads only nodes below s, which the fill has already written, so
+    // the memory nodes the fill has not reached are never read before
+    // finish(), which writes the sentinel into them (a full heap, the common
+    // case, never stores them at all).
     __device__ __forceinline__ void init(uint64_t sentinel)
     {
         count = 0;
         rkey = n1 = n2 = n3 = n4 = sentinel;
-        const ulonglong2 ss = make_ulonglong2(sentinel, sentinel);
-        const uint32_t pairs = k > 5 ? (k - 5 + 1) / 2 : 0;  // pairs of nodes 5..k-1
-        const uint32_t lds_pairs = TAIL ? min(pairs, (kLdsTop - 5 + 1) / 2) : pairs;
-        for (uint32_t p = 0; p < lds_pairs; ++p) *reinterpret_cast<ulonglong2*>(keys + p * pstride) = ss;
-        for (uint32_t p = lds_pairs; p < pairs; ++p)
-            *reinterpret_cast<ulonglong2*>(gkeys + (p - lds_pairs) * gpstride) = ss;
     }
 
     // precondition: key < worst, worst = the root (the sentinel until k keys
@@ -601,8 +599,10 @@ struct Heap4
     }
 
     // sort ascending (entries never filled keep the sentinel)
-    __device__ __forceinline__ void finish(uint64_t)
+    __device__ __forceinline__ void finish(uint64_t sentinel)
     {
+        // memory nodes 5..k-1-count were never reached by the fill
+        for (uint32_t i = 5; i + count < k; ++i) store(i, sentinel);
         uint32_t end = k - 1;
         for (; end > 4; --end)
         {
