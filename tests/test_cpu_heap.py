@@ -1,0 +1,39 @@
+"""CPU check of the search kernels' 4-ary heap.
+
+The Heap4 struct is copied out of point_geometric_features_amd/csrc/
+pgeof_search.hip into tests/cpp/heap4_check.cpp, compiled with the host
+compiler (with address and undefined-behaviour checks) and run: for every k
+up to 130, with and without the global tail for k > 21, partially and fully
+filled heaps over poisoned memory must come out as the k smallest keys in
+order (see the .cpp file for the details).
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "..", "point_geometric_features_amd", "csrc", "pgeof_search.hip")
+HARNESS = os.path.join(HERE, "cpp", "heap4_check.cpp")
+
+
+def _heap4_struct() -> str:
+    s = open(SRC).read()
+    start = s.index("template <bool TAIL>\nstruct Heap4\n{")
+    end = s.index("\n};\n", start) + 4
+    return s[start:end]
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")
+def test_heap4_on_cpu(tmp_path):
+    code = open(HARNESS).read()
+    assert "// @HEAP4@" in code
+    cpp = tmp_path / "heap4_check.cpp"
+    cpp.write_text(code.replace("// @HEAP4@", _heap4_struct()))
+    exe = tmp_path / "heap4_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-o", str(exe), str(cpp)], check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 mismatches" in r.stdout
