@@ -34,7 +34,7 @@ EPS = 1e-3
 
 def build() -> str:
     """Compile the C oracle with its Makefile; returns the library path."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
     return os.path.join(_HERE, "libpgeof_oracle.so")
 
 

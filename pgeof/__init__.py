@@ -1,0 +1,27 @@
+"""Drop-in import name of the reference package.
+
+The reference is imported as ``pgeof`` (``src/pgeof/__init__.py:1-9``), which
+re-exports its extension module's names.  This package does the same over the
+gfx950 implementation, so ``import pgeof`` works unchanged for code written
+against the reference.
+"""
+from point_geometric_features_amd import (  # noqa: F401
+    EFeatureID,
+    compute_features,
+    compute_features_multiscale,
+    compute_features_optimal,
+    compute_features_selected,
+    knn_search,
+    radius_search,
+)
+from point_geometric_features_amd.pgeof_ext import *  # noqa: F401,F403  (EFeatureID values)
+
+__all__ = [
+    "EFeatureID",
+    "compute_features",
+    "compute_features_multiscale",
+    "compute_features_optimal",
+    "knn_search",
+    "radius_search",
+    "compute_features_selected",
+]

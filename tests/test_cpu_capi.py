@@ -124,3 +124,12 @@ def test_heap_tail_workspace():
         capi.set_tuning("heap_tail", 1)
     pairs = 2 * ((k - 21 + 3) // 4)
     assert with_tail - without >= (100_000 + 256) * pairs * 16
+
+
+def test_pgeof_import_name_is_a_drop_in():
+    # the reference is imported as `pgeof` (src/pgeof/__init__.py:1-9)
+    import pgeof as ref_name
+
+    for name in pgeof.__all__:
+        assert getattr(ref_name, name) is getattr(pgeof, name)
+    assert int(ref_name.Eigentropy) == 13 and int(ref_name.Linearity) == 0
