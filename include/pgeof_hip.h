@@ -151,12 +151,14 @@ int pgeof_compute_features_optimal_device(const float* xyz, int64_t n_xyz, const
 /* Serving loop (no counterpart in the reference; the reference processes one  */
 /* cloud per call on the CPU).  Requests are point clouds of any size; each    */
 /* runs pgeof_knn_features_device with k = knn on its own stream; up to        */
-/* n_slots run concurrently, admitted FIFO while the sum of their HBM          */
-/* footprints (pgeof_server_footprint) stays within hbm_budget_bytes.          */
+/* n_slots run concurrently.  The server's device memory is one pool of        */
+/* hbm_budget_bytes allocated at creation; a request is admitted (FIFO) when   */
+/* a contiguous range of its footprint (pgeof_server_footprint) is free, and   */
+/* returns it on completion: no allocation happens while serving.             */
 /* ------------------------------------------------------------------------ */
 typedef struct pgeof_server pgeof_server;
 
-/* Create on the current device; NULL on failure. */
+/* Create on the current device (allocates the pool); NULL on failure. */
 pgeof_server* pgeof_server_create(int n_slots, size_t hbm_budget_bytes, uint32_t knn);
 
 /* Enqueue a request.  xyz (n,3) f32 host; feat_out (n,11) f32 host;

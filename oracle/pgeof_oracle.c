@@ -637,14 +637,44 @@ static void kd_select(const float *pts, uint32_t *perm, int64_t lo, int64_t hi, 
     }
 }
 
-static int32_t kd_build_rec(kdtree_t *T, int64_t lo, int64_t hi)
+/* Node count of the subtree over m points (leaves hold at most 10; a node of
+   m > 10 points splits at m / 2).  Only O(log n) distinct sizes occur, so the
+   counts are memoised in a small table filled before the parallel build. */
+typedef struct
 {
-    if (T->n_nodes == T->cap_nodes)
-    {
-        T->cap_nodes *= 2;
-        T->nodes = (kdnode_t *)realloc(T->nodes, sizeof(kdnode_t) * (size_t)T->cap_nodes);
-    }
-    int32_t id = (int32_t)T->n_nodes++;
+    int64_t m[256], c[256];
+    int n;
+} kd_counts_t;
+
+static int64_t kd_count(kd_counts_t *C, int64_t m)
+{
+    for (int i = 0; i < C->n; ++i)
+        if (C->m[i] == m) return C->c[i];
+    const int64_t c = m > 10 ? 1 + kd_count(C, m / 2) + kd_count(C, m - m / 2) : 1;
+    if (C->n < 256) C->m[C->n] = m, C->c[C->n] = c, C->n++;
+    return c;
+}
+
+typedef struct
+{
+    kdtree_t *T;
+    const kd_counts_t *C;
+    int64_t lo, hi;
+    int32_t id;
+    int spawn; /* levels below this node that still fork a thread */
+} kd_task_t;
+
+/* Builds the subtree over perm[lo, hi) at node id; its nodes take the ids
+   id .. id + count - 1 in preorder (node, left subtree, right subtree), the
+   layout a serial depth-first build produces, so the tree does not depend
+   on the number of threads.  The top `spawn` levels build their left
+   subtree on a new thread. */
+static void *kd_build_task(void *arg)
+{
+    kd_task_t *K = (kd_task_t *)arg;
+    kdtree_t *T = K->T;
+    const int64_t lo = K->lo, hi = K->hi;
+    const int32_t id = K->id;
     kdnode_t nd;
     memset(&nd, 0, sizeof nd);
     nd.lo = lo, nd.hi = hi, nd.left = nd.right = -1;
@@ -666,27 +696,41 @@ static int32_t kd_build_rec(kdtree_t *T, int64_t lo, int64_t hi)
         kd_select(T->pts, T->perm, lo, hi, mid, dim);
         nd.dim = dim;
         nd.split = T->pts[(size_t)T->perm[mid] * 3 + dim];
-        T->nodes[id] = nd;
-        int32_t l = kd_build_rec(T, lo, mid);
-        int32_t r = kd_build_rec(T, mid, hi);
-        T->nodes[id].left = l;
-        T->nodes[id].right = r;
-        return id;
+        nd.left = id + 1;
+        nd.right = id + 1 + (int32_t)kd_count((kd_counts_t *)K->C, mid - lo);
+        kd_task_t L = {T, K->C, lo, mid, nd.left, K->spawn - 1};
+        kd_task_t R = {T, K->C, mid, hi, nd.right, K->spawn - 1};
+        pthread_t th;
+        const int forked = K->spawn > 0 && pthread_create(&th, NULL, kd_build_task, &L) == 0;
+        if (!forked) kd_build_task(&L);
+        kd_build_task(&R);
+        if (forked) pthread_join(th, NULL);
     }
     T->nodes[id] = nd;
-    return id;
+    return NULL;
 }
 
-static void kd_build(kdtree_t *T, const float *pts, int64_t n)
+/* KD-tree with leaves of at most 10 points and median splits along the
+   widest box axis; n_threads > 1 builds the subtrees of the top levels in
+   parallel (same tree as the serial build). */
+static void kd_build(kdtree_t *T, const float *pts, int64_t n, int n_threads)
 {
     T->pts = pts;
     T->n = n;
     T->perm = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
     for (int64_t i = 0; i < n; ++i) T->perm[i] = (uint32_t)i;
-    T->cap_nodes = 1024;
-    T->n_nodes = 0;
+    kd_counts_t C;
+    C.n = 0;
+    T->n_nodes = n > 0 ? kd_count(&C, n) : 0;
+    T->cap_nodes = T->n_nodes > 0 ? T->n_nodes : 1;
     T->nodes = (kdnode_t *)malloc(sizeof(kdnode_t) * (size_t)T->cap_nodes);
-    if (n > 0) kd_build_rec(T, 0, n);
+    int spawn = 0;
+    while (spawn < 12 && (1 << spawn) < 2 * n_threads) ++spawn;
+    if (n > 0)
+    {
+        kd_task_t root = {T, &C, 0, n, 0, spawn};
+        kd_build_task(&root);
+    }
 }
 
 static void kd_free(kdtree_t *T)
@@ -870,7 +914,7 @@ int oracle_knn(const float *data, int64_t n_data, const float *query, int64_t n_
     memset(&J, 0, sizeof J);
     kdtree_t T;
     memset(&T, 0, sizeof T);
-    if (use_kdtree) kd_build(&T, data, n_data);
+    if (use_kdtree) kd_build(&T, data, n_data, n_threads);
     J.T = &T, J.data = data, J.n_data = n_data, J.query = query, J.n_query = n_query, J.cap = knn;
     J.idx_out = idx, J.d_out = d, J.brute = !use_kdtree;
     run_nn(&J, n_threads);
@@ -888,7 +932,7 @@ int oracle_radius(const float *data, int64_t n_data, const float *query, int64_t
     memset(&J, 0, sizeof J);
     kdtree_t T;
     memset(&T, 0, sizeof T);
-    if (use_kdtree) kd_build(&T, data, n_data);
+    if (use_kdtree) kd_build(&T, data, n_data, n_threads);
     J.T = &T, J.data = data, J.n_data = n_data, J.query = query, J.n_query = n_query, J.cap = max_knn;
     J.radius = 1, J.r2 = radius * radius;
     J.idx_out = (uint32_t *)idx, J.d_out = d, J.count_out = counts, J.brute = !use_kdtree;
@@ -1013,7 +1057,7 @@ int oracle_knn_features_timed(const float *xyz, int64_t n, const uint32_t *sampl
     if ((int64_t)k > n) return 1;
     double t0 = now_s();
     kdtree_t T;
-    kd_build(&T, xyz, n);
+    kd_build(&T, xyz, n, n_threads);
     double t1 = now_s();
     float *q = (float *)malloc(sizeof(float) * 3 * (size_t)(n_sample > 0 ? n_sample : 1));
     for (int64_t i = 0; i < n_sample; ++i)

@@ -363,7 +363,7 @@ struct Heap
     uint32_t k;
     uint32_t count;  // entries held; the heap property holds once count == k
 
-    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return keys[i * stride]; }
+    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return keys[(size_t)i * stride]; }
 
     // put `key` at index i of heap [0, n) and sift it down
     __device__ __forceinline__ void sift_from(uint32_t i, uint64_t key, uint32_t n)
@@ -372,11 +372,11 @@ struct Heap
         {
             const uint32_t l = 2 * i + 1;
             if (l >= n) break;
-            uint64_t ck = keys[l * stride];
+            uint64_t ck = keys[(size_t)l * stride];
             uint32_t c = l;
             if (l + 1 < n)
             {
-                const uint64_t rk = keys[(l + 1) * stride];
+                const uint64_t rk = keys[(size_t)(l + 1) * stride];
                 if (rk > ck)
                 {
                     ck = rk;
@@ -384,15 +384,15 @@ struct Heap
                 }
             }
             if (ck <= key) break;
-            keys[i * stride] = ck;
+            keys[(size_t)i * stride] = ck;
             i = c;
         }
-        keys[i * stride] = key;
+        keys[(size_t)i * stride] = key;
     }
 
     __device__ __forceinline__ void heapify()
     {
-        for (int i = (int)k / 2 - 1; i >= 0; --i) sift_from((uint32_t)i, keys[i * stride], k);
+        for (int i = (int)k / 2 - 1; i >= 0; --i) sift_from((uint32_t)i, keys[(size_t)i * stride], k);
     }
 
     // precondition: key < worst (worst = sentinel until the heap is full)
@@ -400,7 +400,7 @@ struct Heap
     {
         if (count < k)
         {
-            keys[count * stride] = key;
+            keys[(size_t)count * stride] = key;
             if (++count == k)
             {
                 heapify();
@@ -419,14 +419,14 @@ struct Heap
     {
         if (count < k)
         {
-            for (uint32_t j = count; j < k; ++j) keys[j * stride] = sentinel;
+            for (uint32_t j = count; j < k; ++j) keys[(size_t)j * stride] = sentinel;
             heapify();
         }
         for (uint32_t end = k - 1; end > 0; --end)
         {
             const uint64_t top_k = keys[0];
-            const uint64_t last = keys[end * stride];
-            keys[end * stride] = top_k;
+            const uint64_t last = keys[(size_t)end * stride];
+            keys[(size_t)end * stride] = top_k;
             sift_from(0, last, end);
         }
     }
@@ -452,7 +452,7 @@ struct Heap4
     uint32_t k;
     uint32_t count;    // keys inserted, up to k
     uint64_t* gkeys;   // this lane's first global pair; pair p at gkeys + p * gpstride
-    uint32_t gpstride; // entries between consecutive global pairs of one lane (2 * lanes of the launch)
+    size_t gpstride;   // entries between consecutive global pairs of one lane (2 * lanes of the launch)
     uint64_t rkey;     // node 0
     uint64_t n1, n2, n3, n4;  // nodes 1..4 (separate scalars, never an indexed array)
 
@@ -464,10 +464,12 @@ struct Heap4
         const uint32_t s = i - 5;
         return (s >> 1) * pstride + (s & 1);
     }
-    __device__ __forceinline__ uint32_t gslot(uint32_t i) const  // global node
+    // global node; 64-bit: (k - 21) / 2 * gpstride passes 2^32 for large
+    // searches (10 M queries at k >= 452)
+    __device__ __forceinline__ size_t gslot(uint32_t i) const
     {
         const uint32_t s = i - (kLdsTop + 1);
-        return (s >> 1) * gpstride + (s & 1);
+        return (size_t)(s >> 1) * gpstride + (s & 1);
     }
     __device__ __forceinline__ uint64_t load(uint32_t i) const  // i >= 5
     {
@@ -521,8 +523,8 @@ struct Heap4
         else
         {
             const uint32_t gg = g - (kLdsTop - 4) / 4;
-            g01 = *reinterpret_cast<const ulonglong2*>(gkeys + (2 * gg) * gpstride);
-            g23 = *reinterpret_cast<const ulonglong2*>(gkeys + (2 * gg + 1) * gpstride);
+            g01 = *reinterpret_cast<const ulonglong2*>(gkeys + (size_t)(2 * gg) * gpstride);
+            g23 = *reinterpret_cast<const ulonglong2*>(gkeys + (size_t)(2 * gg + 1) * gpstride);
         }
         uint64_t mk = g01.x;
         uint32_t mj = 0;
@@ -748,7 +750,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
         {
             // nodes past 20 in global memory (lane-interleaved pairs)
             heap.gkeys = a.gheap_keys + (size_t)t * 2;
-            heap.gpstride = 2u * gridDim.x * (uint32_t)BLOCK;
+            heap.gpstride = (size_t)2 * gridDim.x * (size_t)BLOCK;
         }
     }
     heap.k = a.k;
@@ -1322,13 +1324,13 @@ struct PairHeap
         {
             const uint32_t l = 2 * i + 1;
             if (l >= n) break;
-            T ck = key[l * stride];
-            uint32_t ci = idx[l * stride];
+            T ck = key[(size_t)l * stride];
+            uint32_t ci = idx[(size_t)l * stride];
             uint32_t c = l;
             if (l + 1 < n)
             {
-                const T rk = key[(l + 1) * stride];
-                const uint32_t ri = idx[(l + 1) * stride];
+                const T rk = key[(size_t)(l + 1) * stride];
+                const uint32_t ri = idx[(size_t)(l + 1) * stride];
                 if (less(ck, ci, rk, ri))
                 {
                     ck = rk;
@@ -1337,12 +1339,12 @@ struct PairHeap
                 }
             }
             if (!less(kk, ii, ck, ci)) break;
-            key[i * stride] = ck;
-            idx[i * stride] = ci;
+            key[(size_t)i * stride] = ck;
+            idx[(size_t)i * stride] = ci;
             i = c;
         }
-        key[i * stride] = kk;
-        idx[i * stride] = ii;
+        key[(size_t)i * stride] = kk;
+        idx[(size_t)i * stride] = ii;
     }
     __device__ __forceinline__ void sort_ascending()
     {
@@ -1350,10 +1352,10 @@ struct PairHeap
         {
             const T tk = key[0];
             const uint32_t ti = idx[0];
-            const T lk = key[end * stride];
-            const uint32_t li = idx[end * stride];
-            key[end * stride] = tk;
-            idx[end * stride] = ti;
+            const T lk = key[(size_t)end * stride];
+            const uint32_t li = idx[(size_t)end * stride];
+            key[(size_t)end * stride] = tk;
+            idx[(size_t)end * stride] = ti;
             sift_root(lk, li, end);
         }
     }
@@ -1457,8 +1459,8 @@ __global__ __launch_bounds__(BLOCK) void k_selected(SelectedArgs a)
     const T r2 = (T)a.r2;
     for (uint32_t j = 0; j < cap; ++j)
     {
-        heap.key[j * heap.stride] = r2;
-        heap.idx[j * heap.stride] = 0xFFFFFFFFu;
+        heap.key[(size_t)j * heap.stride] = r2;
+        heap.idx[(size_t)j * heap.stride] = 0xFFFFFFFFu;
     }
     const T* __restrict__ X = reinterpret_cast<const T*>(a.xyz);
     const float4 qf = a.grid.pts[t];
@@ -1499,7 +1501,7 @@ __global__ __launch_bounds__(BLOCK) void k_selected(SelectedArgs a)
     heap.sort_ascending();
     uint32_t cnt = 0;
     for (uint32_t j = 0; j < cap; ++j)
-        if (heap.idx[j * heap.stride] != 0xFFFFFFFFu) ++cnt;
+        if (heap.idx[(size_t)j * heap.stride] != 0xFFFFFFFFu) ++cnt;
     T* out = reinterpret_cast<T*>(a.out) + (size_t)qi * a.n_sel;
     if (cnt < 2 || a.max_knn == 0)
     {
@@ -1512,7 +1514,7 @@ __global__ __launch_bounds__(BLOCK) void k_selected(SelectedArgs a)
     double o[3] = {0.0, 0.0, 0.0};
     for (uint32_t j = 0; j < m; ++j)
     {
-        const uint32_t pi = heap.idx[j * heap.stride];
+        const uint32_t pi = heap.idx[(size_t)j * heap.stride];
         const double px = (double)X[(size_t)pi * 3 + 0], py = (double)X[(size_t)pi * 3 + 1],
                      pz = (double)X[(size_t)pi * 3 + 2];
         if (j == 0) o[0] = px, o[1] = py, o[2] = pz;

@@ -2,25 +2,33 @@
 // knn_search(k) -> CSR -> compute_features on the device
 // (pgeof_knn_features_device).
 //
-// - `n_slots` requests are in flight at once, each on its own HIP stream
-//   with its own device workspace, so their kernels and copies overlap on
-//   the GPU (continuous batching: a request is admitted as soon as a slot
-//   and enough of the HBM budget are free, not when a batch completes).
-// - Admission is FIFO by HBM footprint: the head request waits until the
-//   sum of the footprints of the requests in flight plus its own fits the
-//   budget; a request larger than the whole budget is rejected at submit.
-// - The grid and the CSR stay in the slot's device memory; the features (and
-//   the CSR when the caller asks for it) are copied straight into the
-//   caller's arrays.
+// - `n_slots` requests are in flight at once, each on its own HIP stream, so
+//   their kernels and copies overlap on the GPU (continuous batching: a
+//   request is admitted as soon as a slot and enough device memory are free,
+//   not when a batch completes).
+// - Device memory is one pool of hbm_budget_bytes, allocated when the server
+//   is created.  An admitted request takes one contiguous range of it (first
+//   fit) for its inputs, outputs and search workspace, and returns it when it
+//   completes, so the server's device memory never exceeds the budget and
+//   the workers never call hipMalloc or hipFree (hipFree would synchronise
+//   the device and stall the other slots).
+// - Admission is FIFO: the head request waits until a range of its
+//   footprint is free; a request larger than the whole budget is rejected at
+//   submit.  With nothing in flight the whole pool is free, so the head
+//   request is always admitted eventually.
+// - The grid and the CSR stay in device memory; the features (and the CSR
+//   when the caller asks for it) are copied straight into the caller's
+//   arrays.
 // - Each request's result does not depend on what else is in flight: every
-//   kernel is deterministic and a slot's buffers are private to it.
+//   kernel is deterministic and a request's memory range is private to it.
 #include <hip/hip_runtime.h>
 
-#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <iterator>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -33,6 +41,27 @@ namespace
 
 using Clock = std::chrono::steady_clock;
 
+inline size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Device memory of one request: xyz, nn, nn_ptr, features, search workspace.
+struct Carve
+{
+    size_t x, nn, ptr, f, ws, total;
+};
+
+Carve carve_of(int64_t n, uint32_t k)
+{
+    const size_t pts = (size_t)n;
+    Carve c;
+    c.x = al(pts * 12);
+    c.nn = al(pts * k * 4);
+    c.ptr = al((pts + 1) * 4);
+    c.f = al(pts * 11 * 4);
+    c.ws = al(pgeof_knn_workspace_size(n, n, k));
+    c.total = c.x + c.nn + c.ptr + c.f + c.ws;
+    return c;
+}
+
 struct Request
 {
     uint64_t id;
@@ -41,6 +70,7 @@ struct Request
     uint32_t* nn_out;  // may be null
     float* feat_out;
     size_t footprint;
+    size_t offset;  // start of the request's range in the pool (set at admission)
     Clock::time_point t_submit;
 };
 
@@ -51,17 +81,58 @@ struct Done
     double latency_ms;
 };
 
-size_t footprint_of(int64_t n, uint32_t k)
+// First-fit allocator of ranges of the device pool (offsets only); callers
+// hold the server mutex.
+class RangePool
 {
-    const size_t pts = (size_t)n;
-    return pts * 12 + pts * k * 4 + (pts + 1) * 4 + pts * 11 * 4 + pgeof_knn_workspace_size(n, n, k);
-}
+  public:
+    void reset(size_t size)
+    {
+        free_.clear();
+        if (size) free_[0] = size;
+    }
+    bool take(size_t bytes, size_t* off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= bytes)
+            {
+                *off = it->first;
+                const size_t rest = it->second - bytes;
+                const size_t at = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                return true;
+            }
+        return false;
+    }
+    void give(size_t off, size_t bytes)
+    {
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first)
+        {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin())
+        {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first)
+            {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    bool fits(size_t bytes) const
+    {
+        for (const auto& kv : free_)
+            if (kv.second >= bytes) return true;
+        return false;
+    }
 
-struct Slot
-{
-    hipStream_t stream = nullptr;
-    char* dev = nullptr;
-    size_t dev_bytes = 0;
+  private:
+    std::map<size_t, size_t> free_;  // offset -> size of each free range
 };
 
 }  // namespace
@@ -71,89 +142,71 @@ struct pgeof_server
     int device = 0;
     uint32_t k = 20;
     size_t budget = 0;
-    size_t in_use = 0;
+    char* pool_base = nullptr;
+    RangePool pool;
     bool stop = false;
     std::mutex mu;
-    std::condition_variable cv_queue;  // queue / budget changes
+    std::condition_variable cv_queue;  // queue / pool changes
     std::condition_variable cv_done;
     std::deque<Request> queue;
     std::deque<Done> done;
     std::vector<std::thread> workers;
-    std::vector<Slot> slots;
-    std::string err;
+    std::vector<hipStream_t> streams;
 };
 
 namespace
 {
 
-int ensure(Slot& s, size_t dev_bytes)
-{
-    if (dev_bytes > s.dev_bytes)
-    {
-        if (s.dev) (void)hipFree(s.dev);
-        s.dev = nullptr;
-        s.dev_bytes = 0;
-        if (hipMalloc(&s.dev, dev_bytes) != hipSuccess) return PGEOF_ENOMEM;
-        s.dev_bytes = dev_bytes;
-    }
-    return PGEOF_OK;
-}
-
-inline size_t al(size_t b) { return (b + 255) & ~size_t(255); }
-
-int run_request(pgeof_server* S, Slot& slot, const Request& r)
+int run_request(pgeof_server* S, hipStream_t st, const Request& r)
 {
     const uint32_t k = S->k;
     const size_t n = (size_t)r.n;
     if (n == 0) return PGEOF_OK;
-    const size_t b_x = al(n * 12), b_nn = al(n * k * 4), b_ptr = al((n + 1) * 4), b_f = al(n * 11 * 4);
-    const size_t b_ws = pgeof_knn_workspace_size(r.n, r.n, k);
-    int rc = ensure(slot, b_x + b_nn + b_ptr + b_f + b_ws + 4096);
-    if (rc) return rc;
-    char* d = slot.dev;
+    const Carve c = carve_of(r.n, k);
+    char* d = S->pool_base + r.offset;
     float* d_x = (float*)d;
-    uint32_t* d_nn = (uint32_t*)(d + b_x);
-    uint32_t* d_ptr = (uint32_t*)(d + b_x + b_nn);
-    float* d_f = (float*)(d + b_x + b_nn + b_ptr);
-    void* d_ws = d + b_x + b_nn + b_ptr + b_f;
-    hipStream_t st = slot.stream;
+    uint32_t* d_nn = (uint32_t*)(d + c.x);
+    uint32_t* d_ptr = (uint32_t*)(d + c.x + c.nn);
+    float* d_f = (float*)(d + c.x + c.nn + c.ptr);
+    void* d_ws = d + c.x + c.nn + c.ptr + c.f;
     // straight from and into the caller's memory: the runtime stages pageable
     // memory itself, pipelined (measured faster than a pinned buffer per slot
     // plus host copies)
-    if (hipMemcpyAsync(d_x, r.xyz, n * 12, hipMemcpyHostToDevice, st) != hipSuccess) return PGEOF_ERUNTIME;
-    rc = pgeof_knn_features_device(d_x, r.n, k, d_nn, d_ptr, nullptr, d_f, d_ws, b_ws, st);
+    int rc = PGEOF_OK;
+    if (hipMemcpyAsync(d_x, r.xyz, n * 12, hipMemcpyHostToDevice, st) != hipSuccess) rc = PGEOF_ERUNTIME;
+    if (!rc) rc = pgeof_knn_features_device(d_x, r.n, k, d_nn, d_ptr, nullptr, d_f, d_ws, c.ws, st);
+    if (!rc && hipMemcpyAsync(r.feat_out, d_f, n * 11 * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = PGEOF_ERUNTIME;
+    if (!rc && r.nn_out && hipMemcpyAsync(r.nn_out, d_nn, n * k * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = PGEOF_ERUNTIME;
+    // every way out waits for the stream: the request is reported only once
+    // nothing queued for it still reads the caller's arrays or its pool range
+    const hipError_t se = hipStreamSynchronize(st);
     if (rc) return rc;
-    if (hipMemcpyAsync(r.feat_out, d_f, n * 11 * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return PGEOF_ERUNTIME;
-    if (r.nn_out && hipMemcpyAsync(r.nn_out, d_nn, n * k * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
-        return PGEOF_ERUNTIME;
-    if (hipStreamSynchronize(st) != hipSuccess) return PGEOF_ERUNTIME;
-    return PGEOF_OK;
+    return se == hipSuccess ? PGEOF_OK : PGEOF_ERUNTIME;
 }
 
 void worker(pgeof_server* S, int idx)
 {
     (void)hipSetDevice(S->device);
-    Slot& slot = S->slots[idx];
+    hipStream_t st = S->streams[idx];
     for (;;)
     {
         Request r;
         {
             std::unique_lock<std::mutex> l(S->mu);
-            S->cv_queue.wait(l, [&] {
-                return S->stop || (!S->queue.empty() && S->in_use + S->queue.front().footprint <= S->budget);
-            });
-            if (S->stop && S->queue.empty()) return;
-            if (S->stop && !(S->in_use + S->queue.front().footprint <= S->budget)) return;
+            S->cv_queue.wait(l, [&] { return S->stop || (!S->queue.empty() && S->pool.fits(S->queue.front().footprint)); });
+            if (S->stop) return;
             r = S->queue.front();
             S->queue.pop_front();
-            S->in_use += r.footprint;
+            if (!S->pool.take(r.footprint, &r.offset)) r.offset = 0;  // cannot happen: fits() was checked under the lock
         }
         S->cv_queue.notify_all();
-        const int rc = run_request(S, slot, r);
+        const int rc = run_request(S, st, r);
         const double ms = std::chrono::duration<double, std::milli>(Clock::now() - r.t_submit).count();
         {
             std::lock_guard<std::mutex> l(S->mu);
-            S->in_use -= r.footprint;
+            S->pool.give(r.offset, r.footprint);
             S->done.push_back(Done{r.id, rc, ms});
         }
         S->cv_queue.notify_all();
@@ -173,11 +226,20 @@ pgeof_server* pgeof_server_create(int n_slots, size_t hbm_budget_bytes, uint32_t
     pgeof_server* S = new pgeof_server();
     S->device = dev;
     S->k = knn;
-    S->budget = hbm_budget_bytes;
-    S->slots.resize((size_t)n_slots);
-    for (auto& s : S->slots)
-        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess)
+    S->budget = hbm_budget_bytes & ~size_t(255);
+    if (S->budget == 0 || hipMalloc(&S->pool_base, S->budget) != hipSuccess)
+    {
+        delete S;
+        return nullptr;
+    }
+    S->pool.reset(S->budget);
+    S->streams.assign((size_t)n_slots, nullptr);
+    for (auto& s : S->streams)
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
         {
+            for (auto& t : S->streams)
+                if (t) (void)hipStreamDestroy(t);
+            (void)hipFree(S->pool_base);
             delete S;
             return nullptr;
         }
@@ -189,7 +251,7 @@ int pgeof_server_submit(pgeof_server* S, uint64_t id, const float* xyz, int64_t 
 {
     if (!S || n < 0 || (n > 0 && (!xyz || !feat_out))) return PGEOF_EINVAL;
     if ((int64_t)S->k > n && n > 0) return PGEOF_EINVAL;
-    Request r{id, xyz, n, nn_out, feat_out, footprint_of(n, S->k), Clock::now()};
+    Request r{id, xyz, n, nn_out, feat_out, carve_of(n, S->k).total, 0, Clock::now()};
     {
         std::lock_guard<std::mutex> l(S->mu);
         if (r.footprint > S->budget) return PGEOF_ENOMEM;
@@ -217,7 +279,7 @@ int pgeof_server_wait(pgeof_server* S, int max, uint64_t* ids, int* status, doub
     return c;
 }
 
-size_t pgeof_server_footprint(int64_t n, uint32_t knn) { return footprint_of(n, knn); }
+size_t pgeof_server_footprint(int64_t n, uint32_t knn) { return carve_of(n, knn).total; }
 
 void pgeof_server_destroy(pgeof_server* S)
 {
@@ -228,12 +290,12 @@ void pgeof_server_destroy(pgeof_server* S)
     }
     S->cv_queue.notify_all();
     for (auto& t : S->workers) t.join();
-    for (auto& s : S->slots)
+    for (auto& s : S->streams)
     {
-        if (s.stream) (void)hipStreamSynchronize(s.stream);
-        if (s.dev) (void)hipFree(s.dev);
-        if (s.stream) (void)hipStreamDestroy(s.stream);
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
     }
+    (void)hipFree(S->pool_base);
     delete S;
 }
 

@@ -158,6 +158,44 @@ def request_sizes(n_requests: int, seed: int = 4, lo: int = 5_000, hi: int = 2_0
     return np.exp(rng.uniform(np.log(lo), np.log(hi), n_requests)).astype(np.int64)
 
 
+def request_stream(n_requests: int, seed: int = 4, base_points: int = 2_000_000, workers: int = 16) -> list:
+    """The serving workload's requests: sizes log-uniform in [5k, 2M]
+    (``request_sizes``), each a square window of one LiDAR-like base scene of
+    ``base_points`` points at the 10 M-point / 200 m density, its points in a
+    seeded random order (a window of a scan is not in spatial order).
+    Windows are cut from the base sorted into 128 x 128 ground tiles, with
+    ``workers`` threads."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    extent = 200.0 * float(np.sqrt(base_points / 10_000_000))
+    base = lidar_scene(base_points, seed=seed, extent=extent)
+    sizes = request_sizes(n_requests, seed=seed)
+    G = 128
+    tx = np.minimum((base[:, 0] / extent * G).astype(np.int64), G - 1).clip(0)
+    ty = np.minimum((base[:, 1] / extent * G).astype(np.int64), G - 1).clip(0)
+    tile = ty * G + tx
+    order = np.argsort(tile, kind="stable")
+    starts = np.searchsorted(tile[order], np.arange(G * G + 1))
+    tiled = base[order]
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i, sz in enumerate(sizes):
+        side = extent * min(1.0, float(np.sqrt(sz / base_points)))
+        cx, cy = rng.uniform(side / 2, extent - side / 2, size=2) if side < extent else (extent / 2, extent / 2)
+        jobs.append((int(sz), float(cx), float(cy), side, seed * 100_003 + i))
+
+    def cut(job):
+        sz, cx, cy, side, s = job
+        x0, x1 = int(max(0, (cx - side / 2) / extent * G)), int(min(G - 1, (cx + side / 2) / extent * G))
+        y0, y1 = int(max(0, (cy - side / 2) / extent * G)), int(min(G - 1, (cy + side / 2) / extent * G))
+        p = np.concatenate([tiled[starts[y * G + x0]:starts[y * G + x1 + 1]] for y in range(y0, y1 + 1)])
+        p = p[(np.abs(p[:, 0] - cx) <= side / 2) & (np.abs(p[:, 1] - cy) <= side / 2)][:sz]
+        return np.ascontiguousarray(p[np.random.default_rng(s).permutation(len(p))])
+
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        return list(ex.map(cut, jobs))
+
+
 def request_cloud(size: int, seed: int) -> np.ndarray:
     """One serving request: a LiDAR-like scene of `size` points at the density
     of the 10 M-point / 200 m scene."""

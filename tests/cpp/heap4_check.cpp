@@ -105,10 +105,33 @@ static int check(std::mt19937_64& rng)
     return bad;
 }
 
+// Offsets of global tail nodes for launches with billions of lanes: the
+// node offset (i - 21) / 2 * gpstride must not wrap at 2^32.
+static int check_tail_offsets()
+{
+    int bad = 0;
+    Heap4<true> h{};
+    for (uint64_t lanes : {1ull << 30, (1ull << 31) - 64, 10000000ull, 4294967040ull})
+    {
+        h.gpstride = 2 * lanes;
+        for (uint32_t i = 21; i < 2000; ++i)
+        {
+            const uint64_t s = i - 21;
+            const uint64_t want = (s >> 1) * (2 * lanes) + (s & 1);
+            if ((uint64_t)h.gslot(i) != want)
+            {
+                if (bad < 10) std::printf("tail offset: lanes=%llu node=%u\n", (unsigned long long)lanes, i);
+                ++bad;
+            }
+        }
+    }
+    return bad;
+}
+
 int main()
 {
     std::mt19937_64 rng(7);
-    const int bad = check<false>(rng) + check<true>(rng);
+    const int bad = check<false>(rng) + check<true>(rng) + check_tail_offsets();
     std::printf("heap4 check: %d mismatches\n", bad);
     return bad != 0;
 }

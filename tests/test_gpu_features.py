@@ -9,7 +9,7 @@ import pytest
 import point_geometric_features_amd as pgeof
 from oracle import oracle as O
 from point_geometric_features_amd import scenes
-from tests.parity import assert_features_close
+from tests.parity import assert_features_close, optimal_disagreement_is_tie
 
 pytestmark = pytest.mark.gpu
 
@@ -169,6 +169,7 @@ def test_optimal_vs_oracle(k_min, k_step, k_min_search):
     agree = g[:, 11] == ref[:, 11]
     # k_opt is an argmin over FP32 eigentropies; near-ties may resolve differently
     assert agree.mean() > 0.995, agree.mean()
+    assert optimal_disagreement_is_tie(xyz, nn, 40, np.nonzero(~agree)[0], g[~agree, 11], ref[~agree, 11])
     kopt = ref[agree, 11].astype(np.int64)
     rows = np.nonzero(agree)[0]
     val = np.zeros((len(rows), 3))

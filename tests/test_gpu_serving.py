@@ -64,3 +64,32 @@ def test_serving_admission_limits():
         srv.submit(2, small, outs[1])
         _drain(srv, 2)
         np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_serving_memory_stays_within_the_budget():
+    """A large request followed by small ones on every slot: the server's
+    device memory is one pool of the budget, so the small requests reuse the
+    large one's range instead of growing per-slot buffers."""
+    import torch
+
+    k = 20
+    big = scenes.lidar_scene(120_000, seed=64, extent=12.0)
+    smalls = [scenes.uniform_cube(4000 + 500 * i, seed=65 + i) for i in range(8)]
+    budget = capi.Server.footprint(len(big), k) + 4096
+    assert sum(capi.Server.footprint(len(s), k) for s in smalls) > capi.Server.footprint(len(big), k) // 8
+    torch.cuda.synchronize()
+    before = torch.cuda.mem_get_info()[0]
+    with capi.Server(4, budget, k) as srv:
+        fb = np.empty((len(big), 11), np.float32)
+        srv.submit(0, big, fb)
+        outs = [np.empty((len(s), 11), np.float32) for s in smalls]
+        for i, s in enumerate(smalls):
+            srv.submit(1 + i, s, outs[i])
+        _drain(srv, 1 + len(smalls))
+        during = torch.cuda.mem_get_info()[0]
+    # the pool is the only device allocation the server makes (streams aside)
+    assert before - during <= budget + (64 << 20)
+    for i, s in enumerate(smalls):
+        idx, _ = pgeof.knn_search(s, s, k)
+        nn, ptr = scenes.knn_csr(idx)
+        np.testing.assert_array_equal(outs[i], pgeof.compute_features(s, nn, ptr))

@@ -237,3 +237,49 @@ def test_plausible_mistakes_are_caught():
     no_eps = good.copy()
     no_eps[2] = s[2] / s[0]
     assert abs(no_eps[2] - good[2]) > 1e-3
+
+
+def test_eigentropy_closed_forms_and_shannon_entropy():
+    """Eigentropy (include/pca.hpp:140-150) against what the mathematics
+    fixes: for lambda = (1, 1, 1) it is ln 3 shifted by the epsilons, for a
+    single non-zero eigenvalue it is -e ln(e + eps) of e = s / (s + eps), and
+    on any spectrum it is the Shannon entropy of the normalised eigenvalues
+    (scipy.stats.entropy, natural log) up to terms of the order of eps.  A
+    log10, a dropped minus sign or a missing epsilon in the sum fails one of
+    these."""
+    from scipy.stats import entropy
+
+    e = 1.0 / (3.0 + EPS)
+    want = -3.0 * e * np.log(e + EPS)
+    for h in (O.eigentropy_np(np.array([[1.0, 1.0, 1.0]]))[0], O.eigentropy32_np(np.array([[1.0, 1.0, 1.0]]))[0]):
+        assert abs(h - want) < 2e-7
+    assert abs(want - np.log(3.0)) < 5e-3  # epsilons aside, ln 3
+    s2 = 0.7
+    es = s2 / (s2 + EPS)
+    want1 = -es * np.log(es + EPS)  # the two zero eigenvalues contribute 0 * ln(eps) = 0
+    assert abs(O.eigentropy_np(np.array([[0.0, 0.0, s2]]))[0] - want1) < 1e-15
+    rng = np.random.default_rng(11)
+    lam = np.sort(rng.uniform(0.05, 4.0, size=(500, 3)), axis=1)
+    h = O.eigentropy_np(lam)
+    hs = entropy(lam.T)
+    assert np.abs(h - hs).max() < 5e-3
+    assert np.abs(O.eigentropy32_np(lam) - h).max() < 1e-6
+
+
+def test_selected_verticality_and_eigentropy_closed_forms():
+    """compute_features_selected's own outputs (include/pca.hpp:284 Verticality
+    = 1 - |v2.z|, :287 Eigentropy) on a rotated box of 8 points whose
+    covariance is R diag(a^2, b^2, c^2) R^T, in the float and the double
+    overload of the oracle: every point's neighbourhood is the whole box."""
+    a, b, c = 0.3, 0.8, 1.6
+    box = np.array([[x, y, z] for x in (-a, a) for y in (-b, b) for z in (-c, c)])
+    R = Rotation.from_euler("xyz", [0.4, -0.7, 0.25]).as_matrix()
+    pts = box @ R.T + [10.0, -3.0, 2.0]
+    lam = np.array([a * a, b * b, c * c])
+    v2 = R[:, 2] * np.sign(R[2, 2])  # major axis, z >= 0 (pca.hpp:96)
+    es = lam / (lam.sum() + EPS)
+    want = {12: 1.0 - abs(v2[2]), 13: -(es * np.log(es + EPS)).sum(), 4: v2[0], 5: v2[1], 6: v2[2]}
+    for dt, tol in ((np.float64, 1e-9), (np.float32, 2e-5)):
+        got = O.compute_features_selected(pts.astype(dt), 100.0, 8, list(want))
+        for j, (fid, w) in enumerate(want.items()):
+            np.testing.assert_allclose(got[:, j], w, atol=tol, err_msg=f"feature {fid} {dt.__name__}")
