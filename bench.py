@@ -35,6 +35,11 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public spec
 
 
+def _progress(msg: str) -> None:
+    """One line per finished phase on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +51,8 @@ def _parse():
     ap.add_argument("--tune", action="append", default=[], help="library tuning knob key=value")
     ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
     ap.add_argument("--no-serving", action="store_true", help="default workload: skip the serving stream")
+    ap.add_argument("--no-autotune", action="store_true", help="time with the default knobs (no autotuner sweep)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg")
     return ap.parse_args()
 
 
@@ -194,6 +201,7 @@ def _serving(args, capi, scenes):
     k = 20
     clouds = [c for c in scenes.request_stream(args.requests, seed=4) if len(c) >= k]
     n_pts = sum(len(c) for c in clouds)
+    _progress(f"serving: {len(clouds)} requests generated ({n_pts} points)")
     budget = 64 << 30
 
     def run(subset, n_slots, arrivals=None, sequential=False):
@@ -235,12 +243,15 @@ def _serving(args, capi, scenes):
 
     run(clouds[:64], 4)  # warm-up (code objects, pool allocation)
     wall_seq, lat_seq = run(clouds, 1, sequential=True)
+    _progress(f"serving: one request at a time {wall_seq:.2f} s")
     wall_cap, lat_cap = run(clouds, 4)
+    _progress(f"serving: closed loop {wall_cap:.2f} s")
     cap_rate = n_pts / wall_cap
     rate_req = 0.8 * cap_rate / (n_pts / len(clouds))
     gaps = np.random.default_rng(5).exponential(1.0 / rate_req, size=len(clouds))
     arrivals = np.cumsum(gaps) - gaps[0]
     wall_open, lat_open = run(clouds, 4, arrivals)
+    _progress(f"serving: open loop {wall_open:.2f} s")
     seq_rate, open_rate = n_pts / wall_seq, n_pts / wall_open
     # "p99-latency-adjusted throughput": points/s divided by the p99 latency
     # (s), i.e. throughput that counts each result only as fast as the
@@ -259,6 +270,40 @@ def _serving(args, capi, scenes):
         "throughput_ratio_open_vs_one_at_a_time": round(open_rate / seq_rate, 3),
         "p99_adjusted_throughput_ratio": round(adj_open / adj_seq, 3),
     }
+
+
+def _graph_leg(torch, step, args, units):
+    """The same step captured once into a HIP graph (torch.cuda.CUDAGraph:
+    the searches plan their grids on the device and queue every launch
+    without a read-back) and replayed: `steps` timed replays, each followed by
+    a synchronisation, as the eager steps."""
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step_on = lambda: step()  # noqa: E731
+        step_on()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step()
+    for _ in range(args.warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+    total = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1e3
+    _progress(f"graph replays: {total / args.steps * 1e3:.3f} ms per step")
+    return {"value": round(units * args.steps / total / 1e6, 3), "ms_per_step": round(total / args.steps * 1e3, 4),
+            "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3),
+                           "p99": round(float(np.percentile(lat_ms, 99)), 3)},
+            "launches_per_step": 1}
 
 
 def _sha(a: np.ndarray) -> str:
@@ -378,6 +423,7 @@ def main():
         return
 
     cpu_leg = None  # callable run after the timed steps: (cpu_baseline, feature_err)
+    tuned = None  # autotuner report of the workload, when it ran
     if wl in ("lidar10m_k20", "lidar10m_k50"):
         k = 20 if wl.endswith("k20") else 50
         n = 10_000_000
@@ -388,10 +434,11 @@ def main():
         nn = torch.empty(n * k, dtype=torch.int32, device=dev)
         ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
         feat = torch.empty((n, 11), dtype=torch.float32, device=dev)
+        tuned = None if args.no_autotune else capi.autotune("knn_features", x, k)
         ws = torch.empty(capi.knn_workspace_size(n, n, k), dtype=torch.uint8, device=dev)
 
         def step():
-            capi.knn_features_device(x, k, nn, ptr, feat, workspace=ws, stream=stream)
+            capi.knn_features_device(x, k, nn, ptr, feat, workspace=ws)
 
         def cpu_leg():
             return _cpu_baseline(xyz, k, args.cpu_sample, nn, feat)
@@ -431,7 +478,7 @@ def main():
         err = torch.zeros(1, dtype=torch.int32, device=dev)
 
         def step():
-            capi.compute_features_device(x, nn, ptr, feat, k_min=1, err=err, stream=stream)
+            capi.compute_features_device(x, nn, ptr, feat, k_min=1, err=err)
 
         kernel = "csr_features"
         bytes_per_step = (60 + 4 * k) * n
@@ -440,42 +487,40 @@ def main():
         n, r, cap, sc = 2_000_000, 0.5, 64, [10, 20, 50, 100]
         xyz = scenes.lidar_scene(n, seed=2, extent=200.0 * np.sqrt(n / 10_000_000))
         cfg.update({"points": n, "radius": r, "max_knn": cap, "scales": sc,
-                    "path": "radius CSR + compute_features_multiscale, device-resident"})
+                    "path": "radius CSR + compute_features_multiscale in one call (pgeof_radius_multiscale_device), "
+                            "device-resident"})
         x = torch.from_numpy(xyz).to(dev)
         nn = torch.empty(n * cap, dtype=torch.int32, device=dev)
         ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
         feat = torch.empty((n, len(sc), 11), dtype=torch.float32, device=dev)
-        ws = torch.empty(capi.radius_csr_workspace_size(n, cap), dtype=torch.uint8, device=dev)
-        err = torch.zeros(1, dtype=torch.int32, device=dev)
-        nnz_box = [0]
+        tuned = None if args.no_autotune else capi.autotune("radius_multiscale", x, cap, radius=r, k_scales=sc)
+        ws = torch.empty(capi.radius_multiscale_workspace_size(n, cap, len(sc)), dtype=torch.uint8, device=dev)
 
         def step():
-            nnz = capi.radius_csr_device(x, r, cap, nn, ptr, workspace=ws, stream=stream)
-            nnz_box[0] = nnz
-            capi.compute_features_multiscale_device(x, nn[:nnz], ptr, sc, feat, err=err, stream=stream)
+            capi.radius_multiscale_device(x, r, cap, sc, nn, ptr, feat, workspace=ws)
 
         def cpu_leg():
             return _cpu_multiscale(xyz, r, cap, sc, nn, ptr, feat, 50_000)
 
-        kernel = "csr_features_multiscale"
+        fused = (tuned["knobs"] if tuned else {"fuse_multiscale": capi.get_tuning("fuse_multiscale")})["fuse_multiscale"]
+        kernel = "radius_multiscale_fused" if fused else "csr_features_multiscale"
         bytes_per_step = None  # set after the first step: 16 n + 4 nnz + 44 S n
         units = n
     elif wl == "optimal_1m":
         n, k = 1_000_000, 100
         xyz = scenes.precision_scene(n, seed=3)
         cfg.update({"points": n, "k": k, "k_min": 10, "k_step": 1, "k_min_search": 10,
-                    "path": "knn k=100 -> compute_features_optimal, device-resident"})
+                    "path": "knn k=100 -> compute_features_optimal in one call (pgeof_knn_optimal_device), "
+                            "device-resident"})
         x = torch.from_numpy(xyz).to(dev)
         nn = torch.empty(n * k, dtype=torch.int32, device=dev)
         ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
-        f11 = torch.empty((n, 11), dtype=torch.float32, device=dev)
         feat = torch.empty((n, 12), dtype=torch.float32, device=dev)
-        ws = torch.empty(capi.knn_workspace_size(n, n, k), dtype=torch.uint8, device=dev)
-        err = torch.zeros(1, dtype=torch.int32, device=dev)
+        tuned = None if args.no_autotune else capi.autotune("knn_optimal", x, k)
+        ws = torch.empty(capi.knn_optimal_workspace_size(n, k), dtype=torch.uint8, device=dev)
 
         def step():
-            capi.knn_features_device(x, k, nn, ptr, f11, workspace=ws, stream=stream)
-            capi.compute_features_optimal_device(x, nn, ptr, feat, 10, 1, 10, err=err, stream=stream)
+            capi.knn_optimal_device(x, k, nn, ptr, feat, 10, 1, 10, workspace=ws)
 
         def cpu_leg():
             return _cpu_optimal(xyz, k, nn, feat, 20_000, (10, 1, 10))
@@ -486,13 +531,15 @@ def main():
     else:
         raise SystemExit(f"unknown workload {wl}")
     cfg["inputs_sha256_16"] = _sha(xyz)
+    _progress(f"{wl}: inputs ready")
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     if wl == "radius_multiscale_2m":
-        bytes_per_step = 16 * n + 4 * nnz_box[0] + 44 * len(sc) * n
-        cfg["nnz"] = int(nnz_box[0])
+        nnz = int(ptr[-1].item())
+        bytes_per_step = 16 * n + 4 * nnz + 44 * len(sc) * n
+        cfg["nnz"] = nnz
 
     capi.profile_reset()
     capi.profile_enable(True)
@@ -508,6 +555,7 @@ def main():
     t1 = time.perf_counter()
     capi.profile_enable(False)
     prof = capi.profile_report()
+    _progress(f"{wl}: {args.steps} timed steps done")
 
     total = t1 - t0
     ms_per_step = total / args.steps * 1e3
@@ -525,6 +573,11 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": kernel, "kernel_ms": round(avg_kernel_ms, 4),
                     "algorithmic_bytes_per_step": int(bytes_per_step)}
+    if tuned is not None:
+        extra["autotune"] = {"default_ms": tuned["default_ms"], "best_ms": tuned["best_ms"],
+                             "candidates": len(tuned["candidates"]), "knobs": tuned["knobs"]}
+    if wl != "host_api_lidar10m_k20" and not args.no_graph:
+        extra["graph"] = _graph_leg(torch, step, args, units)
     extra["kernels_ms_per_step"] = {kk: round(v[1] / args.steps, 4) for kk, v in sorted(prof.items())}
     extra["kernel_launches_per_step"] = {kk: round(v[0] / args.steps, 2) for kk, v in sorted(prof.items())}
     lat_ms = np.array(lat) * 1e3
@@ -534,6 +587,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and cpu_leg is not None:
         cpu, ferr = cpu_leg()
+        _progress(f"{wl}: CPU baseline done")
         extra["feature_err_vs_fp64_oracle"] = ferr
         if wl in ("lidar10m_k20", "lidar10m_k50"):
             extra["cpu_scipy_knn"] = _cpu_scipy(xyz, k, args.cpu_sample)

@@ -95,9 +95,10 @@ int pgeof_compute_features_selected_f64(const double* xyz, int64_t n, double sea
 /* Device-memory entry points.  All pointers are device pointers on the       */
 /* current HIP device; work is ordered on `stream` (a hipStream_t, NULL = the */
 /* null stream).  `workspace` may be NULL: the call then takes and returns    */
-/* its scratch with hipMallocAsync/hipFreeAsync on `stream`.  The search      */
-/* entry points read the point cloud's bounding box back to the host, so they */
-/* synchronise `stream` once or twice before their last kernel is queued.     */
+/* its scratch with hipMallocAsync/hipFreeAsync on `stream`.  The searches    */
+/* size their grids on the device and never synchronise `stream`: given a     */
+/* workspace (and, for pgeof_radius_csr_device, nnz_out = NULL) a call only   */
+/* queues kernels, so it can be captured into a HIP graph.                    */
 /* ------------------------------------------------------------------------ */
 
 /* Workspace bytes for the search entry points below. */
@@ -125,11 +126,40 @@ int pgeof_knn_features_device(const float* xyz, int64_t n, uint32_t knn, uint32_
 
 /* radius_search(xyz, xyz, r, max_knn) -> CSR of the hits (README.md:158-159:
  * nn_ptr = [0, cumsum(count)], nn = idx[idx >= 0]).  nn must hold n*max_knn
- * entries; nn_ptr (n+1).  *nnz_out (host) receives nn_ptr[n]. */
+ * entries; nn_ptr (n+1).  If nnz_out (host) is non-NULL the call synchronises
+ * `stream` and stores nn_ptr[n] there; with NULL it stays asynchronous. */
 size_t pgeof_radius_csr_workspace_size(int64_t n, uint32_t max_knn);
 int pgeof_radius_csr_device(const float* xyz, int64_t n, float search_radius, uint32_t max_knn, uint32_t* nn,
                             uint32_t* nn_ptr, int64_t* nnz_out, void* workspace, size_t workspace_bytes,
                             void* stream);
+
+/* radius_search(xyz, xyz, r, max_knn) -> CSR (as pgeof_radius_csr_device) ->
+ * compute_features_multiscale(xyz, nn, nn_ptr, k_scales)
+ * (include/pgeof.hpp:159-211, README.md:158-159) in one call, with no
+ * read-back: capturable into a HIP graph when a workspace of
+ * pgeof_radius_multiscale_workspace_size bytes is given.  The features come
+ * from a CSR kernel after the search, or (tuning knob "fuse_multiscale") from
+ * the search kernel itself, off each row's sorted hits.  k_scales is a host
+ * array (its values travel as kernel arguments).  nn (n*max_knn) u32,
+ * nn_ptr (n+1) u32, features (n, n_scales, 11) f32, equal bit for bit to the
+ * two-call path.  EINVAL as pgeof_radius_csr_device and
+ * pgeof_compute_features_multiscale. */
+size_t pgeof_radius_multiscale_workspace_size(int64_t n, uint32_t max_knn, int64_t n_scales);
+int pgeof_radius_multiscale_device(const float* xyz, int64_t n, float search_radius, uint32_t max_knn,
+                                   const uint32_t* k_scales, int64_t n_scales, uint32_t* nn, uint32_t* nn_ptr,
+                                   float* features, void* workspace, size_t workspace_bytes, void* stream);
+
+/* knn_search(xyz, xyz, knn) -> CSR (nn_ptr = arange*knn) ->
+ * compute_features_optimal(xyz, nn, nn_ptr, k_min, k_step, k_min_search)
+ * (include/pgeof.hpp:243-310, README.md:136-141) in one call: the feature
+ * kernel visits the rows in the search's spatial order.  Workspace:
+ * pgeof_knn_workspace_size(n, n, knn) + 256 bytes; with it the call only
+ * queues kernels (capturable into a HIP graph).  features (n, 12) f32, equal
+ * bit for bit to the two-call path.  EINVAL as pgeof_knn_features_device and
+ * pgeof_compute_features_optimal. */
+int pgeof_knn_optimal_device(const float* xyz, int64_t n, uint32_t knn, uint32_t k_min, uint32_t k_step,
+                             uint32_t k_min_search, uint32_t* nn, uint32_t* nn_ptr, float* features, void* workspace,
+                             size_t workspace_bytes, void* stream);
 
 /* Device forms of the CSR feature drivers.  If err is non-NULL (a device u32)
  * the call is fully asynchronous and ORs 1 (bad neighbour index) / 2 (bad
@@ -202,10 +232,28 @@ void pgeof_server_destroy(pgeof_server* server);
  * nodes 5..20 in LDS, 0: the whole heap in LDS while it fits), "staging"
  * (wave-shared LDS staging of cell rows: 0 never, 1 radius search and,
  * for k >= 32, the budgeted kNN pass, 2 radius search and the budgeted kNN
- * pass at any k).
+ * pass at any k), "fuse_multiscale" (pgeof_radius_multiscale_device: 1 the
+ * features in the search kernel, 0 in a CSR kernel after it).
  * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);
 double pgeof_get_tuning(const char* key);
+
+/* Autotuner.  Times the workload on the given cloud (device pointer xyz, n
+ * points) with the current knobs and with a sweep of the knobs that move its
+ * speed (grid resolution, block size through the heaps' LDS budget, row
+ * staging, and for "radius_multiscale" whether the search kernel computes the
+ * features), `reps` timed runs each (median), on `stream`.  The fastest set
+ * is kept for the workload's search family ("knn_features" and "knn_optimal":
+ * kNN, knn = k; "radius_multiscale": radius, max_knn = k, scales k_scales)
+ * and its size class (ceil(log2 n)): later calls with that k and size class
+ * use it, workspace sizes included, until pgeof_autotune_clear.  It allocates
+ * its own buffers and synchronises: run it before serving, not inside a
+ * graph.  report (if non-NULL) receives a JSON summary (times in ms of every
+ * candidate, the chosen knob set).  Knobs change speed only, never results. */
+int pgeof_autotune(const char* workload, const float* xyz, int64_t n, uint32_t k, float radius,
+                   const uint32_t* k_scales, int64_t n_scales, int reps, void* stream, char* report,
+                   size_t report_len);
+void pgeof_autotune_clear(void);
 
 /* When enabled, every kernel launch is bracketed by HIP events on its stream.
  * pgeof_profile_report writes {"kernel": [launches, total_ms], ...} as JSON

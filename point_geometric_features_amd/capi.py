@@ -48,10 +48,16 @@ _SIGS = {
     "pgeof_knn_features_device": (ctypes.c_int, [_P, _I64, _U32, _P, _P, _P, _P, _P, _SZ, _P]),
     "pgeof_radius_csr_device": (ctypes.c_int, [_P, _I64, _F, _U32, _P, _P, ctypes.POINTER(_I64), _P, _SZ, _P]),
     "pgeof_compute_features_device": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _U64, _P, _P, _P]),
+    "pgeof_radius_multiscale_workspace_size": (_SZ, [_I64, _U32, _I64]),
+    "pgeof_radius_multiscale_device": (ctypes.c_int, [_P, _I64, _F, _U32, _P, _I64, _P, _P, _P, _P, _SZ, _P]),
+    "pgeof_knn_optimal_device": (ctypes.c_int, [_P, _I64, _U32, _U32, _U32, _U32, _P, _P, _P, _P, _SZ, _P]),
     "pgeof_compute_features_multiscale_device": (ctypes.c_int,
                                                  [_P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P]),
     "pgeof_compute_features_optimal_device": (ctypes.c_int,
                                               [_P, _I64, _P, _I64, _P, _I64, _U32, _U32, _U32, _P, _P, _P]),
+    "pgeof_autotune": (ctypes.c_int, [ctypes.c_char_p, _P, _I64, _U32, _F, _P, _I64, ctypes.c_int, _P,
+                                      ctypes.c_char_p, _SZ]),
+    "pgeof_autotune_clear": (None, []),
     "pgeof_set_tuning": (ctypes.c_int, [ctypes.c_char_p, _D]),
     "pgeof_get_tuning": (_D, [ctypes.c_char_p]),
     "pgeof_profile_enable": (None, [ctypes.c_int]),
@@ -109,6 +115,24 @@ def get_tuning(key: str) -> float:
     return float(lib().pgeof_get_tuning(key.encode()))
 
 
+def autotune(workload: str, xyz, k: int, radius: float = 0.0, k_scales=(), reps: int = 3, stream=None) -> dict:
+    """pgeof_autotune on a device cloud `xyz` (torch tensor (n, 3) float32):
+    times the workload under a sweep of knob sets, keeps the fastest for its
+    family, k and size class, and returns the JSON report as a dict."""
+    import numpy as np
+
+    sc = np.ascontiguousarray(k_scales, dtype=np.uint32)
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().pgeof_autotune(workload.encode(), _ptr(xyz), xyz.shape[0], int(k), float(radius),
+                               sc.ctypes.data if len(sc) else None, len(sc), int(reps), _stream(stream), buf,
+                               len(buf)))
+    return json.loads(buf.value.decode())
+
+
+def autotune_clear() -> None:
+    lib().pgeof_autotune_clear()
+
+
 def profile_enable(on: bool = True) -> None:
     lib().pgeof_profile_enable(1 if on else 0)
 
@@ -157,12 +181,41 @@ def knn_features_device(xyz, k, nn, nn_ptr, features, sqr_dist=None, workspace=N
                                           _ptr(features), ws, wsb, _stream(stream)))
 
 
-def radius_csr_device(xyz, r, max_knn, nn, nn_ptr, workspace=None, stream=None) -> int:
+def radius_csr_device(xyz, r, max_knn, nn, nn_ptr, workspace=None, stream=None, sync_nnz=True):
+    """Returns nnz (synchronising the stream), or None with sync_nnz=False
+    (asynchronous: nnz = nn_ptr[n] on the device)."""
     ws, wsb = (_ptr(workspace), workspace.numel() * workspace.element_size()) if workspace is not None else (None, 0)
     nnz = _I64(0)
     check(lib().pgeof_radius_csr_device(_ptr(xyz), xyz.shape[0], float(r), int(max_knn), _ptr(nn), _ptr(nn_ptr),
-                                        ctypes.byref(nnz), ws, wsb, _stream(stream)))
-    return int(nnz.value)
+                                        ctypes.byref(nnz) if sync_nnz else None, ws, wsb, _stream(stream)))
+    return int(nnz.value) if sync_nnz else None
+
+
+def radius_multiscale_device(xyz, r, max_knn, k_scales, nn, nn_ptr, features, workspace=None, stream=None):
+    """pgeof_radius_multiscale_device: radius CSR + multiscale features, the
+    features computed in the search kernel."""
+    import numpy as np
+
+    sc = np.ascontiguousarray(k_scales, dtype=np.uint32)
+    ws, wsb = (_ptr(workspace), workspace.numel() * workspace.element_size()) if workspace is not None else (None, 0)
+    check(lib().pgeof_radius_multiscale_device(_ptr(xyz), xyz.shape[0], float(r), int(max_knn), sc.ctypes.data,
+                                               len(sc), _ptr(nn), _ptr(nn_ptr), _ptr(features), ws, wsb,
+                                               _stream(stream)))
+
+
+def radius_multiscale_workspace_size(n: int, max_knn: int, n_scales: int) -> int:
+    return int(lib().pgeof_radius_multiscale_workspace_size(n, max_knn, n_scales))
+
+
+def knn_optimal_device(xyz, k, nn, nn_ptr, features, k_min=1, k_step=1, k_min_search=1, workspace=None, stream=None):
+    """pgeof_knn_optimal_device: kNN CSR + optimal-k features in one call."""
+    ws, wsb = (_ptr(workspace), workspace.numel() * workspace.element_size()) if workspace is not None else (None, 0)
+    check(lib().pgeof_knn_optimal_device(_ptr(xyz), xyz.shape[0], int(k), int(k_min), int(k_step), int(k_min_search),
+                                         _ptr(nn), _ptr(nn_ptr), _ptr(features), ws, wsb, _stream(stream)))
+
+
+def knn_optimal_workspace_size(n: int, k: int) -> int:
+    return knn_workspace_size(n, n, k) + 256
 
 
 def compute_features_device(xyz, nn, nn_ptr, features, k_min=1, err=None, stream=None):

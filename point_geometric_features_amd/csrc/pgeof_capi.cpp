@@ -65,24 +65,92 @@ struct Tuning
     double csr_order = 1.0;             // CSR features: visit large inputs' rows in spatial order
     double heap_tail = 1.0;             // search heaps: nodes past 20 in global memory (k > 21)
     double staging = 1.0;               // LDS staging of cell rows: 0 never, 1 radius + budgeted kNN at k >= 32, 2 always
+    double fuse_multiscale = 0.0;       // radius + multiscale: features in the search kernel (1) or a CSR kernel after it (0)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
+
+// name -> member, for pgeof_set_tuning / pgeof_get_tuning and the autotuner
+struct KnobName
+{
+    const char* name;
+    double Tuning::*field;
+};
+const KnobName kKnobs[] = {
+    {"knn_occupancy", &Tuning::knn_occupancy},
+    {"knn_cell_points_max", &Tuning::knn_cell_points_max},
+    {"grid_cells_per_point", &Tuning::grid_cells_per_point},
+    {"lds_budget", &Tuning::lds_budget},
+    {"radius_cell_factor", &Tuning::radius_cell_factor},
+    {"grid_levels", &Tuning::grid_levels},
+    {"near_first", &Tuning::near_first},
+    {"budget_cells", &Tuning::budget_cells},
+    {"budget_shells", &Tuning::budget_shells},
+    {"budget_levels", &Tuning::budget_levels},
+    {"grid_level_ratio", &Tuning::grid_level_ratio},
+    {"csr_order", &Tuning::csr_order},
+    {"heap_tail", &Tuning::heap_tail},
+    {"staging", &Tuning::staging},
+    {"fuse_multiscale", &Tuning::fuse_multiscale},
+};
+
+double* knob(Tuning& t, const std::string& name)
+{
+    for (const KnobName& k : kKnobs)
+        if (name == k.name) return &(t.*(k.field));
+    return nullptr;
+}
 
 // Every entry point works on one snapshot of the knobs, taken when it starts,
 // so that a concurrent pgeof_set_tuning cannot make, e.g., the grid sizing
 // disagree with the workspace carved for it within one call.
 thread_local const Tuning* t_snap = nullptr;
 
+// Knob sets chosen by the autotuner (pgeof_autotune) per search family, k and
+// size class (ceil(log2 n)): a call whose sizes match takes that set instead
+// of the global knobs.
+enum
+{
+    kFamKnn = 0,
+    kFamRadius = 1
+};
+struct TuneKey
+{
+    int family;
+    uint32_t k;
+    int size_class;
+    bool operator<(const TuneKey& o) const
+    {
+        if (family != o.family) return family < o.family;
+        if (k != o.k) return k < o.k;
+        return size_class < o.size_class;
+    }
+};
+std::map<TuneKey, Tuning> g_tuned;  // guarded by g_tune_mu
+
+int size_class(int64_t n)
+{
+    int c = 0;
+    while (c < 62 && (int64_t(1) << c) < n) ++c;
+    return c;
+}
+
 struct TuningSnapshot
 {
     Tuning v;
     const Tuning* prev;
-    TuningSnapshot() : prev(t_snap)
+    TuningSnapshot() : TuningSnapshot(-1, 0, 0) {}
+    // family >= 0: the autotuned set for (family, k, size class of n) if one exists
+    TuningSnapshot(int family, int64_t n, uint32_t k) : prev(t_snap)
     {
         {
             std::lock_guard<std::mutex> l(g_tune_mu);
             v = g_tune;
+            if (family >= 0 && !prev)
+            {
+                auto it = g_tuned.find(TuneKey{family, k, size_class(n)});
+                if (it != g_tuned.end()) v = it->second;
+            }
         }
         if (!prev) t_snap = &v;
     }
@@ -297,7 +365,7 @@ struct SearchWs
     float* dbound;     // per slot of a budgeted pass: bound of the k-th distance of a deferred query
     float* blist;      // bounds of the compacted deferred queries
     float* blist2;
-    Level* levels;     // device copy of the level table
+    LevelTable* tab;   // the grid pyramid, planned on the device
 };
 
 constexpr uint32_t kBboxBlocks = 1024;
@@ -353,11 +421,11 @@ size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool csr)
     const uint64_t G = grid_cap(n) + 1;
     size_t b = 0;
     b += al(kBboxBlocks * 6 * sizeof(float)) + al(64) + al(64);
-    b += al(G * 4) + al((2 * G + 64) * 4) + al(scan_partials_needed(G) * 4);
+    b += al(G * 4) + al((2 * G + 64) * 4) + al(scan_partials_needed(G + 64) * 4);
     b += 2 * al(n * 4) + kMaxLevels * al(n * 16);
     b += 2 * al(m * 4) + al(m * 16);
     b += 2 * al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4) + 2 * al(m * 16) + 3 * al(m * 4);
-    b += al(kMaxLevels * sizeof(Level));
+    b += al(sizeof(LevelTable));
     const HeapPlan hp = heap_plan(k);
     if (const size_t hb = heap_global_bytes(hp, m, k)) b += al(hb);
     if (csr) b += al(m * k * 4) + al((m + 1) * 4) + al(scan_partials_needed(m + 1) * 4);
@@ -373,7 +441,7 @@ SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool csr)
     w.sumsq = ar.take<unsigned long long>(8);
     w.count = ar.take<uint32_t>(G);
     w.cs_all = ar.take<uint32_t>(2 * G + 64);
-    w.scan_part = ar.take<uint32_t>(scan_partials_needed(G));
+    w.scan_part = ar.take<uint32_t>(scan_partials_needed(G + 64));
     w.cell = ar.take<uint32_t>(n);
     w.rank = ar.take<uint32_t>(n);
     for (int l = 0; l < kMaxLevels; ++l) w.pts[l] = ar.take<float4>(n);
@@ -388,7 +456,7 @@ SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool csr)
     w.dbound = ar.take<float>(m);
     w.blist = ar.take<float>(m);
     w.blist2 = ar.take<float>(m);
-    w.levels = ar.take<Level>(kMaxLevels);
+    w.tab = ar.take<LevelTable>(1);
     const HeapPlan hp = heap_plan(k);
     if (const size_t hb = heap_global_bytes(hp, m, k)) w.gheap_keys = ar.take<uint64_t>(hb / 8);
     if (csr)
@@ -404,6 +472,7 @@ SearchWs carve(Arena& ar, uint64_t n, uint64_t m, uint32_t k, bool csr)
 int readback(void* host, const void* dev, size_t bytes, hipStream_t s)
 {
     void* pin = pinned_scratch();
+    static_assert(sizeof(LevelTable) <= 4096, "level table read-back");
     if (!pin || bytes > 4096) return fail(PGEOF_ERUNTIME, "pinned staging buffer unavailable");
     HIP_TRY(hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -411,138 +480,98 @@ int readback(void* host, const void* dev, size_t bytes, hipStream_t s)
     return PGEOF_OK;
 }
 
+// Cell-start table of level l inside cs_all (2 * (cap + 1) + 64 entries):
+// level 0 gets cap + 1 entries, level l > 0 about (cap + 1) / 2^l (a coarser
+// level that does not fit is not built), all 16-byte aligned.
+void level_tables(uint32_t* cs_all, uint64_t cap, uint32_t* cs[kMaxLevels], uint64_t room[kMaxLevels])
+{
+    uint64_t off = 0;
+    for (int l = 0; l < kMaxLevels; ++l)
+    {
+        const uint64_t r = l == 0 ? ((cap + 1 + 3) & ~uint64_t(3)) : ((((cap + 1) >> l) + 4 + 3) & ~uint64_t(3));
+        cs[l] = cs_all + off;
+        room[l] = r;
+        off += r;
+    }
+}
+
 // Sort `data` into a pyramid of uniform grids (cell edge h_0 * ratio^l) and,
 // when query != data, sort the queries into level 0.  kNN sizes h_0 from the
 // target occupancy (k used for sizing); radius searches use one level with
-// cells of radius_cell_factor * radius.
+// cells of radius_cell_factor * radius.  The sizes are planned on the device
+// (k_plan_probe, k_plan_levels): every launch is queued without a read-back,
+// the kernels take their grid from the level table and the launches are
+// sized for the largest grid the workspace allows.
 int build_grid(const float* data, uint32_t n, const float* query, uint32_t m, uint32_t k, float radius, bool is_radius,
                SearchWs& w, SearchArgs& sa, const float4** qs_out, hipStream_t s)
 {
-    const uint32_t nbb = std::max<uint32_t>(1, std::min<uint32_t>(kBboxBlocks, (n + 255) / 256));
-    HIP_TRY(launch_bbox(data, n, w.bbox_partial, nbb, w.bbox_out, s));
-    float bb[6];
-    int rc = readback(bb, w.bbox_out, sizeof bb, s);
-    if (rc) return rc;
-    bool finite = true;
-    for (int i = 0; i < 3; ++i)
-        if (!(bb[i] <= bb[3 + i])) finite = false;
-    if (!finite)
-        for (int i = 0; i < 3; ++i) bb[i] = 0.0f, bb[3 + i] = 0.0f;
-    double ext[3];
-    for (int a = 0; a < 3; ++a) ext[a] = std::max(0.0, (double)bb[3 + a] - (double)bb[a]);
-    const double emax = std::max(ext[0], std::max(ext[1], ext[2]));
-    const uint64_t cap = grid_cap(n);
-    const double hcap = h_for_cap(ext, cap);
-    double h;
     const Tuning t = tuning();
-    if (!(emax > 0.0))
-        h = 1.0;
-    else if (is_radius)
+    const uint64_t cap = grid_cap(n);
+    PlanArgs p{};
+    p.n = n;
+    p.k = k;
+    p.radius = radius;
+    p.is_radius = is_radius ? 1 : 0;
+    p.cap = cap;
+    p.max_levels = is_radius ? 1 : std::max(1, std::min(kMaxLevels, (int)t.grid_levels));
+    p.knn_occupancy = t.knn_occupancy;
+    p.knn_cell_points_max = t.knn_cell_points_max;
+    p.radius_cell_factor = t.radius_cell_factor;
+    p.level_ratio = t.grid_level_ratio;
+    p.probe_stride = std::max<uint32_t>(1, std::min<uint32_t>(64, n >> 20));
+    level_tables(w.cs_all, cap, p.cs, p.cs_room);
+    for (int l = 0; l < kMaxLevels; ++l) p.pts[l] = w.pts[l];
+
+    const uint32_t nbb = std::max<uint32_t>(1, std::min<uint32_t>(kBboxBlocks, (n + 255) / 256));
+    HIP_TRY(launch_bbox(data, n, w.bbox_partial, nbb, w.tab->bb, s));
+    HIP_TRY(launch_plan_probe(p, w.tab, s));
+    if (!is_radius) HIP_TRY(launch_probe(data, n, p.probe_stride, w.tab, w.count, cap + 1, s));
+    HIP_TRY(launch_plan_levels(p, w.tab, s));
+    for (int l = 0; l < p.max_levels; ++l)
     {
-        double r = std::fabs((double)radius);
-        h = std::isfinite(r) && r > 0.0 ? r * t.radius_cell_factor : emax;
-        h = std::max(h, hcap);
-    }
-    else
-    {
-        // target occupancy T; first guess assumes a uniformly filled box,
-        // corrected by the measured point-weighted occupancy assuming the
-        // points lie on 2-D surfaces (LiDAR); clamped by the grid-size cap.
-        const double T = std::max(0.75, std::min(t.knn_occupancy * (double)k, t.knn_cell_points_max));
-        double vol = 1.0;
-        for (int a = 0; a < 3; ++a) vol *= std::max(ext[a], emax * 1e-3);
-        double h1 = std::cbrt(vol * T / std::max<double>(n, 1.0));
-        h1 = std::max(h1, hcap);
-        GridParams g1 = make_grid(bb, h1);
-        // point-weighted occupancy at h1, estimated from every stride-th
-        // point: with sampling fraction p, E[sum c_s^2] = p^2 sum c^2 +
-        // p (1 - p) n
-        const uint32_t stride = std::max<uint32_t>(1, std::min<uint32_t>(64, n >> 20));
-        const double ns = (double)((n + stride - 1) / stride), pf = 1.0 / stride;
-        HIP_TRY(launch_fill_u32(w.count, g1.G, 0u, s));
-        HIP_TRY(launch_hist_probe(data, n, stride, g1, w.count, s));
-        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(w.sumsq), 2, 0u, s));
-        HIP_TRY(launch_sumsq(w.count, g1.G, w.sumsq, s));
-        unsigned long long ss = 0;
-        rc = readback(&ss, w.sumsq, sizeof ss, s);
-        if (rc) return rc;
-        const double m1 = std::max(1.0, ((double)ss - (1.0 - pf) * ns) / (pf * std::max(ns, 1.0)));
-        h = h1;
-        if (m1 > T) h = h1 * std::sqrt(T / m1);
-        h = std::max(h, hcap);
-    }
-    // levels
-    const int max_levels = is_radius ? 1 : std::max(1, std::min(kMaxLevels, (int)t.grid_levels));
-    uint64_t cs_used = 0;
-    const uint64_t cs_room = 2 * (cap + 1) + 64;
-    sa.n_levels = 0;
-    for (int l = 0; l < max_levels; ++l)
-    {
-        const double hl = h * std::pow(std::max(2.0, std::min(8.0, t.grid_level_ratio)), (double)l);
-        const GridParams gp = make_grid(bb, hl);
-        if (l > 0 && (cs_used + ((gp.G + 1 + 3) & ~uint64_t(3)) > cs_room || sa.lv[l - 1].g[0] * (uint64_t)sa.lv[l - 1].g[1] *
-                                                              (uint64_t)sa.lv[l - 1].g[2] <= 64))
-            break;
-        uint32_t* csl = w.cs_all + cs_used;
-        cs_used += (gp.G + 1 + 3) & ~uint64_t(3);  // keep every level's table 16-byte aligned
-        HIP_TRY(launch_fill_u32(w.count, gp.G + 1, 0u, s));
+        // counts straight into the level's table, scanned in place
+        uint32_t* csl = p.cs[l];
+        const uint64_t* G = &w.tab->G[l];
+        HIP_TRY(launch_fill_u32_dyn(csl, p.cs_room[l], G, 1, 0u, s));
         if (l == 0)
-            HIP_TRY(launch_hist(data, n, gp, w.count, w.cell, w.rank, s));
+            HIP_TRY(launch_hist(data, n, w.tab, 0, csl, w.cell, w.rank, s));
         else  // from the level-0 order: runs of equal cells, one atomic each
-            HIP_TRY(launch_hist_sorted(w.pts[0], n, gp, w.count, w.cell, w.rank, s));
-        HIP_TRY(launch_scan_exclusive(w.count, csl, gp.G + 1, w.scan_part, s));
+            HIP_TRY(launch_hist_sorted(w.pts[0], n, w.tab, l, csl, w.cell, w.rank, s));
+        HIP_TRY(launch_scan_exclusive_dyn(csl, csl, p.cs_room[l], G, 1, w.scan_part, s));
         if (l == 0)
-            HIP_TRY(launch_scatter(data, n, w.cell, w.rank, csl, w.pts[l], s));
+            HIP_TRY(launch_scatter(data, n, w.cell, w.rank, csl, w.pts[0], w.tab, 0, s));
         else
-            HIP_TRY(launch_scatter_sorted(w.pts[0], n, w.cell, w.rank, csl, w.pts[l], s));
-        Level& L = sa.lv[l];
-        for (int a = 0; a < 3; ++a)
-        {
-            L.lo[a] = gp.lo[a];
-            L.g[a] = gp.g[a];
-        }
-        L.h = gp.h;
-        L.inv_h = gp.inv_h;
-        L.cs = csl;
-        L.pts = w.pts[l];
-        sa.n_levels = l + 1;
+            HIP_TRY(launch_scatter_sorted(w.pts[0], n, w.cell, w.rank, csl, w.pts[l], w.tab, l, s));
     }
-    const Level& L0 = sa.lv[0];
     if (query == data && m == n)
         *qs_out = w.pts[0];
     else
     {
-        GridParams g0;
-        for (int a = 0; a < 3; ++a)
-        {
-            g0.lo[a] = L0.lo[a];
-            g0.g[a] = L0.g[a];
-        }
-        g0.h = L0.h;
-        g0.inv_h = L0.inv_h;
-        g0.G = (uint64_t)L0.g[0] * L0.g[1] * L0.g[2];
-        HIP_TRY(launch_fill_u32(w.count, g0.G + 1, 0u, s));
-        HIP_TRY(launch_hist(query, m, g0, w.count, w.qcell, w.qrank, s));
-        HIP_TRY(launch_scan_exclusive(w.count, w.count, g0.G + 1, w.scan_part, s));
-        HIP_TRY(launch_scatter(query, m, w.qcell, w.qrank, w.count, w.qs, s));
+        const uint64_t* G0 = &w.tab->G[0];
+        HIP_TRY(launch_fill_u32_dyn(w.count, cap + 1, G0, 1, 0u, s));
+        HIP_TRY(launch_hist(query, m, w.tab, 0, w.count, w.qcell, w.qrank, s));
+        HIP_TRY(launch_scan_exclusive_dyn(w.count, w.count, cap + 1, G0, 1, w.scan_part, s));
+        HIP_TRY(launch_scatter(query, m, w.qcell, w.qrank, w.count, w.qs, w.tab, 0, s));
         *qs_out = w.qs;
     }
-    sa.levels = w.levels;
-    HIP_TRY(launch_store_levels(sa, w.levels, s));
+    sa.tab = w.tab;
+    sa.base_level = 0;
     return PGEOF_OK;
 }
 
 // Device search core shared by the host and device entry points.
 int run_search(int mode, const float* data, uint32_t n, const float* query, uint32_t m, uint32_t k, float radius,
                void* out_idx, float* out_d2, float* out_feat, uint32_t* out_count, Arena& ar, hipStream_t s,
-               SearchWs* ws_out = nullptr)
+               SearchWs* ws_out = nullptr, const uint32_t* scales = nullptr, uint32_t n_scales = 0)
 {
-    const bool csr = mode == kModeRadiusCsr;
+    const bool csr = mode == kModeRadiusCsr || mode == kModeRadiusMulti;
     SearchWs w = carve(ar, n, m, k, csr);
     if (ar.off > ar.size) return fail(PGEOF_ENOMEM, "workspace too small");
     SearchArgs a{};
+    a.scales = scales;
+    a.n_scales = n_scales;
     const float4* qs = nullptr;
-    const bool is_radius = mode == kModeRadius || mode == kModeRadiusCsr;
+    const bool is_radius = mode == kModeRadius || mode == kModeRadiusCsr || mode == kModeRadiusMulti;
     int rc = build_grid(data, n, query, m, k, radius, is_radius, w, a, &qs, s);
     if (rc) return rc;
     a.data = data;
@@ -568,53 +597,55 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     {
         // Budgeted passes, finest level first: a query whose k-th neighbour
         // lies beyond +-2 cells of the level is flagged and, compacted and in
-        // sorted order, searched again one level coarser (cells 4x larger).
-        // What is left after the coarsest level goes to the unbudgeted pass,
-        // which picks a level per query.
+        // sorted order, searched again one level coarser (cells 2x larger).
+        // What is left after the last budgeted level goes to the unbudgeted
+        // pass, which picks a level per query.
         // A deferred query carries the k-th distance found so far (when its
         // heap was full) to the next pass, which scans only that ball.
+        // The number of queries left after pass l is a device-side counter
+        // (tab->count[l + 1]): the later passes are launched for up to m
+        // queries and take their count from it.
         const float4* list = qs;
         const float* blist = nullptr;
-        uint32_t n_list = m;
+        const uint64_t* n_dev = nullptr;  // queries in `list` (nullptr: all m)
         float4* next = w.qlist;
         float* bnext = w.blist;
-        const int budget_levels = std::max(1, std::min(a.n_levels, (int)tuning().budget_levels));
-        for (int l = 0; l < budget_levels && n_list; ++l)
+        const int max_levels = std::max(1, std::min(kMaxLevels, (int)tuning().grid_levels));
+        const int budget_levels = std::max(1, std::min(max_levels, (int)tuning().budget_levels));
+        for (int l = 0; l < budget_levels; ++l)
         {
             SearchArgs b = a;
-            b.lv[0] = a.lv[l];
+            b.base_level = l;
             b.queries = list;
-            b.n_query = n_list;
+            b.n_query = m;
+            b.n_query_dev = n_dev;
             b.defer_flag = w.flags;
             b.query_bound = blist;
             b.defer_bound = w.dbound;
             HIP_TRY(launch_search(mode, b, hp.block, hp.lds, hp.global, true, s));
-            HIP_TRY(launch_fill_u32(w.flags + n_list, 1, 0u, s));
-            HIP_TRY(launch_scan_exclusive(w.flags, w.foff, (uint64_t)n_list + 1, w.fpart, s));
-            uint32_t n_def = 0;
-            rc = readback(&n_def, w.foff + n_list, 4, s);
-            if (rc) return rc;
-            if (n_def) HIP_TRY(launch_compact_queries(list, n_list, w.flags, w.foff, next, w.dbound, bnext, s));
+            if (n_dev)
+                HIP_TRY(launch_put_u32(w.flags, n_dev, 0u, s));
+            else
+                HIP_TRY(launch_fill_u32(w.flags + m, 1, 0u, s));
+            HIP_TRY(launch_scan_exclusive_dyn(w.flags, w.foff, (uint64_t)m + 1, n_dev, 1, w.fpart, s));
+            uint64_t* n_next = &w.tab->count[l + 1];
+            HIP_TRY(launch_compact_queries(list, m, n_dev, w.flags, w.foff, next, w.dbound, bnext, n_next, s));
             list = next;
             blist = bnext;
-            n_list = n_def;
+            n_dev = n_next;
             next = next == w.qlist ? w.qlist2 : w.qlist;
             bnext = bnext == w.blist ? w.blist2 : w.blist;
         }
-        if (n_list)
-        {
-            // slowest queries first, lanes of similar cost in each wave
-            HIP_TRY(launch_order_by_bound(list, blist, n_list, w.count, w.flags, w.foff, w.scan_part, next, bnext, s));
-            list = next;
-            blist = bnext;
-            SearchArgs b = a;
-            b.queries = list;
-            b.n_query = n_list;
-            b.defer_flag = nullptr;
-            b.query_bound = blist;
-            b.defer_bound = nullptr;
-            HIP_TRY(launch_search(mode, b, hp.block, hp.lds, hp.global, false, s));
-        }
+        // slowest queries first, lanes of similar cost in each wave
+        HIP_TRY(launch_order_by_bound(list, blist, m, n_dev, w.count, w.flags, w.foff, w.scan_part, next, bnext, s));
+        SearchArgs b = a;
+        b.queries = next;
+        b.n_query = m;
+        b.n_query_dev = n_dev;
+        b.defer_flag = nullptr;
+        b.query_bound = bnext;
+        b.defer_bound = nullptr;
+        HIP_TRY(launch_search(mode, b, hp.block, hp.lds, hp.global, false, s));
     }
     else
         HIP_TRY(launch_search(mode, a, hp.block, hp.lds, hp.global, false, s));
@@ -659,10 +690,13 @@ int err_from_word(uint32_t w)
     return PGEOF_OK;
 }
 
-// Device CSR driver: modes 0 plain, 1 multiscale, 2 optimal.
+// Device CSR driver: modes 0 plain, 1 multiscale, 2 optimal.  The row order
+// of large inputs takes its scratch from `order_ws` (order_ws_bytes >=
+// csr_order_bytes(rows)) when given, else stream-ordered from the pool.
 int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn, const uint32_t* nn_ptr,
                int64_t n_nn_ptr, uint32_t k_min, uint32_t k_step, uint32_t k_min_search, const uint32_t* d_scales,
-               uint32_t n_scales, float* out, uint32_t* err, hipStream_t s)
+               uint32_t n_scales, float* out, uint32_t* err, hipStream_t s, void* order_ws = nullptr,
+               size_t order_ws_bytes = 0)
 {
     CsrArgs a{};
     a.xyz = xyz;
@@ -689,12 +723,21 @@ int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, in
     void* tmp = nullptr;
     if (mode != 1 && a.n_points >= 65536 && n_xyz > 0 && tuning().csr_order > 0.5)
     {
-        HIP_TRY(hipMallocAsync(&tmp, csr_order_bytes(a.n_points), s));
+        void* ows = order_ws;
+        if (ows)
+        {
+            if (order_ws_bytes < csr_order_bytes(a.n_points)) return fail(PGEOF_ENOMEM, "row-order scratch too small");
+        }
+        else
+        {
+            HIP_TRY(hipMallocAsync(&tmp, csr_order_bytes(a.n_points), s));
+            ows = tmp;
+        }
         uint32_t* ord[1] = {nullptr};
-        hipError_t e = launch_csr_order(a, tmp, ord, s);
+        hipError_t e = launch_csr_order(a, ows, ord, s);
         if (e != hipSuccess)
         {
-            (void)hipFreeAsync(tmp, s);
+            if (tmp) (void)hipFreeAsync(tmp, s);
             return hip_fail(e, "launch_csr_order");
         }
         a.order = ord[0];
@@ -808,17 +851,22 @@ int selected_host(const T* xyz, int64_t n, T search_radius, uint32_t max_knn, co
     const float4* qs = nullptr;
     rc = build_grid(d_xf, (uint32_t)n, d_xf, (uint32_t)n, cap, (float)search_radius, true, w, sa, &qs, s);
     if (rc) return rc;
+    // this host entry point synchronises anyway: the kernel takes level 0 by value
+    LevelTable tab;
+    rc = readback(&tab, w.tab, sizeof tab, s);
+    if (rc) return rc;
+    const Level& L0 = tab.lv[0];
     GridView gv;
     for (int ax = 0; ax < 3; ++ax)
     {
-        gv.p.lo[ax] = sa.lv[0].lo[ax];
-        gv.p.g[ax] = sa.lv[0].g[ax];
+        gv.p.lo[ax] = L0.lo[ax];
+        gv.p.g[ax] = L0.g[ax];
     }
-    gv.p.h = sa.lv[0].h;
-    gv.p.inv_h = sa.lv[0].inv_h;
-    gv.p.G = (uint64_t)sa.lv[0].g[0] * sa.lv[0].g[1] * sa.lv[0].g[2];
-    gv.cell_start = sa.lv[0].cs;
-    gv.pts = sa.lv[0].pts;
+    gv.p.h = L0.h;
+    gv.p.inv_h = L0.inv_h;
+    gv.p.G = (uint64_t)L0.g[0] * L0.g[1] * L0.g[2];
+    gv.cell_start = L0.cs;
+    gv.pts = L0.pts;
     SelectedArgs a{};
     a.grid = gv;
     a.xyz = d_x;
@@ -892,58 +940,17 @@ int pgeof_set_tuning(const char* key, double v)
     std::lock_guard<std::mutex> l(g_tune_mu);
     const std::string k = key ? key : "";
     if (!(v >= 0.0) || !std::isfinite(v)) return fail(PGEOF_EINVAL, "tuning value must be non-negative");
-    if (k == "knn_occupancy")
-        g_tune.knn_occupancy = v;
-    else if (k == "knn_cell_points_max")
-        g_tune.knn_cell_points_max = v;
-    else if (k == "grid_cells_per_point")
-        g_tune.grid_cells_per_point = v;
-    else if (k == "lds_budget")
-        g_tune.lds_budget = v;
-    else if (k == "radius_cell_factor")
-        g_tune.radius_cell_factor = v;
-    else if (k == "grid_levels")
-        g_tune.grid_levels = v;
-    else if (k == "near_first")
-        g_tune.near_first = v;
-    else if (k == "budget_cells")
-        g_tune.budget_cells = v;
-    else if (k == "budget_shells")
-        g_tune.budget_shells = v;
-    else if (k == "budget_levels")
-        g_tune.budget_levels = v;
-    else if (k == "grid_level_ratio")
-        g_tune.grid_level_ratio = v;
-    else if (k == "csr_order")
-        g_tune.csr_order = v;
-    else if (k == "heap_tail")
-        g_tune.heap_tail = v;
-    else if (k == "staging")
-        g_tune.staging = v;
-    else
-        return fail(PGEOF_EINVAL, "unknown tuning key " + k);
+    double* f = knob(g_tune, k);
+    if (!f) return fail(PGEOF_EINVAL, "unknown tuning key " + k);
+    *f = v;
     return PGEOF_OK;
 }
 
 double pgeof_get_tuning(const char* key)
 {
     std::lock_guard<std::mutex> l(g_tune_mu);
-    const std::string k = key ? key : "";
-    if (k == "knn_occupancy") return g_tune.knn_occupancy;
-    if (k == "knn_cell_points_max") return g_tune.knn_cell_points_max;
-    if (k == "grid_cells_per_point") return g_tune.grid_cells_per_point;
-    if (k == "lds_budget") return g_tune.lds_budget;
-    if (k == "radius_cell_factor") return g_tune.radius_cell_factor;
-    if (k == "grid_levels") return g_tune.grid_levels;
-    if (k == "near_first") return g_tune.near_first;
-    if (k == "budget_cells") return g_tune.budget_cells;
-    if (k == "budget_shells") return g_tune.budget_shells;
-    if (k == "budget_levels") return g_tune.budget_levels;
-    if (k == "grid_level_ratio") return g_tune.grid_level_ratio;
-    if (k == "csr_order") return g_tune.csr_order;
-    if (k == "heap_tail") return g_tune.heap_tail;
-    if (k == "staging") return g_tune.staging;
-    return -1.0;
+    const double* f = knob(g_tune, key ? key : "");
+    return f ? *f : -1.0;
 }
 
 void pgeof_profile_enable(int on) { g_prof.store(on ? 1 : 0); }
@@ -999,7 +1006,7 @@ int pgeof_profile_report(char* buf, size_t len)
 int pgeof_knn_search(const float* data, int64_t n_data, const float* query, int64_t n_query, uint32_t knn,
                      uint32_t* indices, float* sqr_dist)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamKnn, n_data, knn);
     int rc = validate_knn(n_data, n_query, knn, false);
     if (rc) return rc;
     if (n_query == 0 || knn == 0) return PGEOF_OK;
@@ -1032,7 +1039,7 @@ int pgeof_knn_search(const float* data, int64_t n_data, const float* query, int6
 int pgeof_radius_search(const float* data, int64_t n_data, const float* query, int64_t n_query,
                         float search_radius, uint32_t max_knn, int32_t* indices, float* sqr_dist)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamRadius, n_data, max_knn);
     int rc = validate_knn(n_data, n_query, max_knn, true);
     if (rc) return rc;
     if (n_query == 0 || max_knn == 0) return PGEOF_OK;
@@ -1116,20 +1123,20 @@ int pgeof_compute_features_selected_f64(const double* xyz, int64_t n, double sea
 // ---------------------------------------------------- device searches ----
 size_t pgeof_knn_workspace_size(int64_t n_data, int64_t n_query, uint32_t knn)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamKnn, n_data, knn);
     return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), knn, false);
 }
 
 size_t pgeof_radius_workspace_size(int64_t n_data, int64_t n_query, uint32_t max_knn)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamRadius, n_data, max_knn);
     return search_ws_bytes((uint64_t)std::max<int64_t>(n_data, 0), (uint64_t)std::max<int64_t>(n_query, 0), max_knn,
                            false);
 }
 
 size_t pgeof_radius_csr_workspace_size(int64_t n, uint32_t max_knn)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamRadius, n, max_knn);
     return search_ws_bytes((uint64_t)std::max<int64_t>(n, 0), (uint64_t)std::max<int64_t>(n, 0), max_knn, true);
 }
 
@@ -1137,7 +1144,7 @@ int pgeof_knn_search_device(const float* data, int64_t n_data, const float* quer
                             uint32_t* indices, float* sqr_dist, void* workspace, size_t workspace_bytes,
                             void* stream)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamKnn, n_data, knn);
     int rc = validate_knn(n_data, n_query, knn, false);
     if (rc) return rc;
     if (n_query == 0 || knn == 0) return PGEOF_OK;
@@ -1154,7 +1161,7 @@ int pgeof_radius_search_device(const float* data, int64_t n_data, const float* q
                                float search_radius, uint32_t max_knn, int32_t* indices, float* sqr_dist,
                                void* workspace, size_t workspace_bytes, void* stream)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamRadius, n_data, max_knn);
     int rc = validate_knn(n_data, n_query, max_knn, true);
     if (rc) return rc;
     if (n_query == 0 || max_knn == 0) return PGEOF_OK;
@@ -1171,7 +1178,7 @@ int pgeof_knn_features_device(const float* xyz, int64_t n, uint32_t knn, uint32_
                               float* sqr_dist, float* features, void* workspace, size_t workspace_bytes,
                               void* stream)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamKnn, n, knn);
     int rc = validate_knn(n, n, knn, false);
     if (rc) return rc;
     if (knn < 1) return fail(PGEOF_EINVAL, "knn should be >= 1");
@@ -1193,7 +1200,7 @@ int pgeof_radius_csr_device(const float* xyz, int64_t n, float search_radius, ui
                             uint32_t* nn_ptr, int64_t* nnz_out, void* workspace, size_t workspace_bytes,
                             void* stream)
 {
-    TuningSnapshot snapshot;
+    TuningSnapshot snapshot(kFamRadius, n, max_knn);
     int rc = validate_knn(n, n, max_knn, true);
     if (rc) return rc;
     if ((uint64_t)n * max_knn > 0xFFFFFFFFull) return fail(PGEOF_EINVAL, "n * max_knn exceeds the uint32 CSR range");
@@ -1222,6 +1229,114 @@ int pgeof_radius_csr_device(const float* xyz, int64_t n, float search_radius, ui
         if (rc) return rc;
         *nnz_out = nnz;
     }
+    return PGEOF_OK;
+}
+
+size_t pgeof_radius_multiscale_workspace_size(int64_t n, uint32_t max_knn, int64_t n_scales)
+{
+    TuningSnapshot snapshot(kFamRadius, n, max_knn);
+    return al((size_t)std::max<int64_t>(n_scales, 0) * 4 + 4) + pgeof_radius_csr_workspace_size(n, max_knn);
+}
+
+int pgeof_radius_multiscale_device(const float* xyz, int64_t n, float search_radius, uint32_t max_knn,
+                                   const uint32_t* k_scales, int64_t n_scales, uint32_t* nn, uint32_t* nn_ptr,
+                                   float* features, void* workspace, size_t workspace_bytes, void* stream)
+{
+    TuningSnapshot snapshot(kFamRadius, n, max_knn);
+    if (n_scales < 0 || (n_scales > 0 && !k_scales) || !check_scales(k_scales, n_scales))
+        return fail(PGEOF_EINVAL, "k_scales should be > 1 and sorted in ascending order");
+    int rc = validate_knn(n, n, max_knn, true);
+    if (rc) return rc;
+    if ((uint64_t)n * max_knn > 0xFFFFFFFFull) return fail(PGEOF_EINVAL, "n * max_knn exceeds the uint32 CSR range");
+    if ((uint64_t)n * (uint64_t)n_scales * 11 >= (1ull << 40)) return fail(PGEOF_EINVAL, "output too large");
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) return PGEOF_OK;
+    if (max_knn == 0)
+    {
+        // no hits: empty rows, every scale zero (k_scales entries are >= 1)
+        HIP_TRY(launch_fill_u32(nn_ptr, (uint64_t)n + 1, 0u, s));
+        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(features), (uint64_t)n * n_scales * 11, 0u, s));
+        return PGEOF_OK;
+    }
+    WsGuard g;
+    Arena ar;
+    rc = acquire_ws(g, workspace, workspace_bytes, pgeof_radius_multiscale_workspace_size(n, max_knn, n_scales), s,
+                    ar);
+    if (rc) return rc;
+    uint32_t* d_sc = ar.take<uint32_t>((size_t)n_scales + 1);  // the scales, then the error word of the CSR kernel
+    HIP_TRY(launch_put_values(d_sc, k_scales, (uint64_t)n_scales, s));
+    const bool fused = tuning().fuse_multiscale > 0.5;
+    SearchWs w;
+    rc = run_search(fused ? kModeRadiusMulti : kModeRadiusCsr, xyz, (uint32_t)n, xyz, (uint32_t)n, max_knn,
+                    search_radius, nullptr, nullptr, features, nullptr, ar, s, &w, d_sc, (uint32_t)n_scales);
+    if (rc) return rc;
+    HIP_TRY(launch_fill_u32(w.counts + n, 1, 0u, s));
+    HIP_TRY(launch_scan_exclusive(w.counts, nn_ptr, (uint64_t)n + 1, w.csr_part, s));
+    HIP_TRY(launch_compact_rows(w.dense, (uint32_t)n, max_knn, w.counts, nn_ptr, nn, s));
+    if (!fused && n_scales)
+    {
+        // the multiscale CSR kernel over the compacted rows (its error word
+        // stays 0: the rows hold search hits only)
+        uint32_t* d_err = d_sc + n_scales;
+        HIP_TRY(launch_fill_u32(d_err, 1, 0u, s));
+        CsrArgs a{};
+        a.xyz = xyz;
+        a.n_xyz = (uint32_t)n;
+        a.nn = nn;
+        a.n_nn = (uint64_t)n * max_knn;  // a bound of nnz: the kernel checks rows against it
+        a.nn_ptr = nn_ptr;
+        a.n_points = (uint32_t)n;
+        a.k_min = a.k_step = a.k_min_search = 1;
+        a.scales = d_sc;
+        a.n_scales = (uint32_t)n_scales;
+        a.out = features;
+        a.err = d_err;
+        HIP_TRY(launch_csr_features(1, a, s));
+    }
+    return PGEOF_OK;
+}
+
+int pgeof_knn_optimal_device(const float* xyz, int64_t n, uint32_t knn, uint32_t k_min, uint32_t k_step,
+                             uint32_t k_min_search, uint32_t* nn, uint32_t* nn_ptr, float* features, void* workspace,
+                             size_t workspace_bytes, void* stream)
+{
+    TuningSnapshot snapshot(kFamKnn, n, knn);
+    int rc = validate_knn(n, n, knn, false);
+    if (rc) return rc;
+    if (knn < 1) return fail(PGEOF_EINVAL, "knn should be >= 1");
+    if (k_min < 1 && k_min_search < 1) return fail(PGEOF_EINVAL, "k_min and k_min_search should be > 1");
+    if (k_step < 1) return fail(PGEOF_EINVAL, "k_step should be >= 1");
+    if ((uint64_t)n * knn > 0xFFFFFFFFull) return fail(PGEOF_EINVAL, "n * knn exceeds the uint32 CSR range");
+    if (n == 0) return PGEOF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    WsGuard g;
+    Arena ar;
+    rc = acquire_ws(g, workspace, workspace_bytes, pgeof_knn_workspace_size(n, n, knn) + al(4), s, ar);
+    if (rc) return rc;
+    uint32_t* d_err = ar.take<uint32_t>(1);
+    HIP_TRY(launch_fill_u32(d_err, 1, 0u, s));
+    SearchWs w;
+    rc = run_search(kModeKnn, xyz, (uint32_t)n, xyz, (uint32_t)n, knn, 0.0f, nn, nullptr, nullptr, nullptr, ar, s,
+                    &w);
+    if (rc) return rc;
+    HIP_TRY(launch_arange_ptr(nn_ptr, (uint32_t)n, knn, s));
+    // rows in the search's spatial order (the sorted points of a self
+    // search), kept in the query-cell array a self search does not use
+    HIP_TRY(launch_order_from_sorted(w.pts[0], (uint32_t)n, w.qcell, s));
+    CsrArgs a{};
+    a.xyz = xyz;
+    a.n_xyz = (uint32_t)n;
+    a.nn = nn;
+    a.n_nn = (uint64_t)n * knn;
+    a.nn_ptr = nn_ptr;
+    a.n_points = (uint32_t)n;
+    a.k_min = k_min;
+    a.k_step = k_step;
+    a.k_min_search = k_min_search;
+    a.out = features;
+    a.err = d_err;
+    a.order = w.qcell;
+    HIP_TRY(launch_csr_features(2, a, s));
     return PGEOF_OK;
 }
 
@@ -1304,6 +1419,178 @@ int pgeof_compute_features_optimal_device(const float* xyz, int64_t n_xyz, const
     if (rc) return rc;
     return csr_device_checked(2, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, k_min, k_step, k_min_search, nullptr, 0,
                               features, err, stream);
+}
+
+// ----------------------------------------------------------- autotuner ----
+int pgeof_autotune(const char* workload, const float* xyz, int64_t n, uint32_t k, float radius,
+                   const uint32_t* k_scales, int64_t n_scales, int reps, void* stream, char* report,
+                   size_t report_len)
+{
+    const std::string wl = workload ? workload : "";
+    int family;
+    if (wl == "knn_features" || wl == "knn_optimal")
+        family = kFamKnn;
+    else if (wl == "radius_multiscale")
+        family = kFamRadius;
+    else
+        return fail(PGEOF_EINVAL, "unknown autotune workload " + wl);
+    if (n < 1 || k < 1) return fail(PGEOF_EINVAL, "autotune needs n >= 1 and k >= 1");
+    reps = std::max(1, std::min(reps, 100));
+    hipStream_t s = (hipStream_t)stream;
+
+    // Candidates: the current knobs first, then a sweep of the knobs that
+    // move the speed of this workload: grid resolution (cells per point,
+    // points per cell or cell edge / radius), block size (through the LDS
+    // budget of the heaps), row staging (wave-shared LDS tiles or per-lane
+    // reads) and, for radius + multiscale, where the features are computed.
+    Tuning base;
+    {
+        std::lock_guard<std::mutex> l(g_tune_mu);
+        base = g_tune;
+    }
+    std::vector<Tuning> cand{base};
+    if (family == kFamKnn)
+    {
+        for (double cpp : {4.0, 8.0, 16.0})
+            for (double occ : {0.25, 0.35, 0.5})
+                for (double lds : {65536.0, 32768.0})
+                    for (double st : {1.0, 2.0})
+                    {
+                        Tuning t = base;
+                        t.grid_cells_per_point = cpp;
+                        t.knn_occupancy = occ;
+                        t.lds_budget = lds;
+                        t.staging = st;
+                        cand.push_back(t);
+                    }
+    }
+    else
+    {
+        for (double rcf : {0.35, 0.5, 0.7, 1.0})
+            for (double fu : {0.0, 1.0})
+                for (double st : {0.0, 1.0})
+                {
+                    Tuning t = base;
+                    t.radius_cell_factor = rcf;
+                    t.fuse_multiscale = fu;
+                    t.staging = st;
+                    cand.push_back(t);
+                }
+    }
+
+    // buffers: outputs, and the largest workspace over the candidates
+    const uint64_t nn_cap = (uint64_t)n * k;
+    const uint64_t n_feat = wl == "knn_features" ? 11 : (wl == "knn_optimal" ? 12 : (uint64_t)n_scales * 11);
+    size_t ws_bytes = 0;
+    for (const Tuning& t : cand)
+    {
+        t_snap = &t;
+        size_t b = family == kFamKnn ? pgeof_knn_workspace_size(n, n, k) + 256
+                                     : pgeof_radius_multiscale_workspace_size(n, k, n_scales);
+        t_snap = nullptr;
+        ws_bytes = std::max(ws_bytes, b);
+    }
+    void *d_nn = nullptr, *d_ptr = nullptr, *d_f = nullptr, *d_ws = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        t_snap = nullptr;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (void* p : {d_nn, d_ptr, d_f, d_ws})
+            if (p) (void)hipFree(p);
+    };
+    hipError_t he = hipMalloc(&d_nn, nn_cap * 4 + 4);
+    if (he == hipSuccess) he = hipMalloc(&d_ptr, ((size_t)n + 1) * 4);
+    if (he == hipSuccess) he = hipMalloc(&d_f, (size_t)n * std::max<uint64_t>(n_feat, 1) * 4);
+    if (he == hipSuccess) he = hipMalloc(&d_ws, ws_bytes);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he != hipSuccess)
+    {
+        cleanup();
+        return hip_fail(he, "autotune buffers");
+    }
+    auto run = [&]() -> int {
+        if (wl == "knn_features")
+            return pgeof_knn_features_device(xyz, n, k, (uint32_t*)d_nn, (uint32_t*)d_ptr, nullptr, (float*)d_f, d_ws,
+                                             ws_bytes, s);
+        if (wl == "knn_optimal")
+            return pgeof_knn_optimal_device(xyz, n, k, 10, 1, 10, (uint32_t*)d_nn, (uint32_t*)d_ptr, (float*)d_f, d_ws,
+                                            ws_bytes, s);
+        return pgeof_radius_multiscale_device(xyz, n, radius, k, k_scales, n_scales, (uint32_t*)d_nn,
+                                              (uint32_t*)d_ptr, (float*)d_f, d_ws, ws_bytes, s);
+    };
+    std::vector<double> ms(cand.size(), 0.0);
+    size_t best = 0;
+    for (size_t c = 0; c < cand.size(); ++c)
+    {
+        t_snap = &cand[c];
+        int rc = run();  // warm-up
+        std::vector<float> tt;
+        for (int r = 0; r < reps && !rc; ++r)
+        {
+            he = hipEventRecord(e0, s);
+            if (he == hipSuccess) rc = run();
+            if (!rc && he == hipSuccess) he = hipEventRecord(e1, s);
+            if (!rc && he == hipSuccess) he = hipEventSynchronize(e1);
+            float v = 0.0f;
+            if (!rc && he == hipSuccess) he = hipEventElapsedTime(&v, e0, e1);
+            if (he != hipSuccess) rc = hip_fail(he, "autotune timing");
+            tt.push_back(v);
+        }
+        t_snap = nullptr;
+        if (rc)
+        {
+            cleanup();
+            return rc;
+        }
+        std::sort(tt.begin(), tt.end());
+        ms[c] = tt[tt.size() / 2];
+        if (ms[c] < ms[best]) best = c;
+    }
+    cleanup();
+    const int sc = size_class(n);
+    {
+        std::lock_guard<std::mutex> l(g_tune_mu);
+        g_tuned[TuneKey{family, k, sc}] = cand[best];
+    }
+    if (report && report_len)
+    {
+        auto knobs_json = [](const Tuning& t) {
+            std::string js = "{";
+            for (size_t i = 0; i < sizeof(kKnobs) / sizeof(kKnobs[0]); ++i)
+            {
+                char tmp[96];
+                std::snprintf(tmp, sizeof tmp, "%s\"%s\": %g", i ? ", " : "", kKnobs[i].name, t.*(kKnobs[i].field));
+                js += tmp;
+            }
+            return js + "}";
+        };
+        std::string js = "{\"workload\": \"" + wl + "\", \"n\": " + std::to_string(n) + ", \"k\": " +
+                         std::to_string(k) + ", \"size_class\": " + std::to_string(sc) + ", \"ms\": [";
+        for (size_t c = 0; c < ms.size(); ++c)
+        {
+            char tmp[32];
+            std::snprintf(tmp, sizeof tmp, "%s%.4f", c ? ", " : "", ms[c]);
+            js += tmp;
+        }
+        char tmp[96];
+        std::snprintf(tmp, sizeof tmp, "], \"chosen\": %zu, \"default_ms\": %.4f, \"best_ms\": %.4f", best, ms[0],
+                      ms[best]);
+        js += tmp;
+        js += ", \"knobs\": " + knobs_json(cand[best]) + ", \"candidates\": [";
+        for (size_t c = 0; c < cand.size(); ++c) js += (c ? ", " : "") + knobs_json(cand[c]);
+        js += "]}";
+        std::strncpy(report, js.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return PGEOF_OK;
+}
+
+void pgeof_autotune_clear(void)
+{
+    std::lock_guard<std::mutex> l(g_tune_mu);
+    g_tuned.clear();
 }
 
 }  // extern "C"

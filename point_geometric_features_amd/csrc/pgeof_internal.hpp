@@ -46,15 +46,48 @@ struct Level
 
 constexpr int kMaxLevels = 4;
 
+// The grid pyramid of one search, planned and written by kernels (bounding
+// box, sizing probe, levels), so that no value travels to the host between
+// the launches of a search.  Level l has cell edge h_0 * ratio^l; a level
+// that was not built has G[l] = 0.
+struct LevelTable
+{
+    Level lv[kMaxLevels];
+    uint64_t G[kMaxLevels];  // cells of level l (0: absent)
+    int n_levels;
+    Level probe;             // kNN sizing probe grid (cs, pts unused)
+    uint64_t probe_G;
+    float bb[8];             // bounding box of the finite data coordinates: lo xyz, hi xyz
+    unsigned long long sumsq;  // probe: sum of squared cell counts
+    double hcap, T, h1;      // planning intermediates (grid-size floor, target occupancy, first guess)
+    uint64_t count[kMaxLevels + 2];  // kNN: queries left after each budgeted pass (device-side work counters)
+};
+
+// Everything the planning kernels need to size the grids of a search.
+struct PlanArgs
+{
+    uint32_t n;  // data points
+    uint32_t k;  // kNN: k (sizes the cells); radius: unused
+    float radius;
+    int is_radius;
+    uint64_t cap;  // largest G of level 0 (grid_cells_per_point * n)
+    int max_levels;
+    double knn_occupancy, knn_cell_points_max, radius_cell_factor, level_ratio;
+    uint32_t probe_stride;
+    uint32_t* cs[kMaxLevels];       // cell-start table of level l (fixed offsets in the workspace)
+    uint64_t cs_room[kMaxLevels];   // entries available there
+    float4* pts[kMaxLevels];        // sorted points of level l
+};
+
 // Arguments of the neighbour-search kernels.
 struct SearchArgs
 {
-    Level lv[kMaxLevels];  // level l has cell edge h_0 * ratio^l
-    const Level* levels;   // the same table in device memory (per-lane level choice reads it)
-    int n_levels;
+    const LevelTable* tab;  // the pyramid (device memory)
+    int base_level;         // budgeted passes and radius searches: the level they scan
     const float* data;      // the caller's data points, (n, 3) float32 (fused features gather from it)
     const float4* queries;  // sorted queries (x, y, z, bits of original query index)
-    uint32_t n_query;
+    uint32_t n_query;       // queries, or their upper bound when n_query_dev is set
+    const uint64_t* n_query_dev;  // device-side query count (deferred passes), or nullptr
     uint32_t k;    // kNN: k; radius: max_knn
     float r2;      // radius: search_radius * search_radius (float)
     float margin;  // absolute distance slack for cell pruning, relative to h (times h_l)
@@ -71,6 +104,8 @@ struct SearchArgs
     int staging;           // wave-shared LDS staging of cell rows: 0 never, 1 radius search, 2 also the budgeted kNN pass
     int budget_cells;      // budgeted pass: largest ball half-width, in cells around the query's cell
     int budget_shells;     // budgeted pass: cubic shells scanned before a query without k candidates is deferred
+    const uint32_t* scales;  // kModeRadiusMulti: the k_scales (device), n_scales of them
+    uint32_t n_scales;
 };
 
 // Simple linear device arena carved out of one allocation.
@@ -89,34 +124,54 @@ struct Arena
 };
 
 // ---- launchers (pgeof_search.hip) ----
+// Sizes given as (n_max, n_dev): the kernels work on min(*n_dev + add, n_max)
+// entries when n_dev is set (a count only the device knows), else on n_max.
 hipError_t launch_bbox(const float* xyz, uint32_t n, float* partial, uint32_t n_partial, float* out6,
                        hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
-hipError_t launch_hist(const float* xyz, uint32_t n, const GridParams& gp, uint32_t* count, uint32_t* cell,
-                       uint32_t* rank, hipStream_t s);
-hipError_t launch_sumsq(const uint32_t* count, uint64_t G, unsigned long long* out, hipStream_t s);
+hipError_t launch_fill_u32_dyn(uint32_t* p, uint64_t n_max, const uint64_t* n_dev, uint64_t add, uint32_t v,
+                               hipStream_t s);
 size_t scan_partials_needed(uint64_t n);
 hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partials,
                                  hipStream_t s);
+hipError_t launch_scan_exclusive_dyn(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint64_t* n_dev,
+                                     uint64_t add, uint32_t* partials, hipStream_t s);
+// Grid planning on the device: pgeof_plan_probe after the bounding box (into
+// tab->bb), pgeof_plan_levels after the probe histogram's sum of squares.
+hipError_t launch_plan_probe(const PlanArgs& p, LevelTable* tab, hipStream_t s);
+hipError_t launch_plan_levels(const PlanArgs& p, LevelTable* tab, hipStream_t s);
+// cell counts of every stride-th point at the probe grid, and their sum of squares
+hipError_t launch_probe(const float* xyz, uint32_t n, uint32_t stride, LevelTable* tab, uint32_t* count,
+                        uint64_t count_room, hipStream_t s);
+// counting sort of points into level l (level 0 from xyz, l > 0 from the
+// level-0 array, already sorted by a finer grid); nothing when the level is absent
+hipError_t launch_hist(const float* xyz, uint32_t n, const LevelTable* tab, int l, uint32_t* count, uint32_t* cell,
+                       uint32_t* rank, hipStream_t s);
+hipError_t launch_hist_sorted(const float4* pts, uint32_t n, const LevelTable* tab, int l, uint32_t* count,
+                              uint32_t* cell, uint32_t* rank, hipStream_t s);
 hipError_t launch_scatter(const float* xyz, uint32_t n, const uint32_t* cell, const uint32_t* rank,
-                          const uint32_t* cell_start, float4* pts, hipStream_t s);
+                          const uint32_t* cell_start, float4* pts, const LevelTable* tab, int l, hipStream_t s);
+hipError_t launch_scatter_sorted(const float4* in, uint32_t n, const uint32_t* cell, const uint32_t* rank,
+                                 const uint32_t* cell_start, float4* out, const LevelTable* tab, int l,
+                                 hipStream_t s);
 hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds_bytes, bool global_heap, bool budget,
                          hipStream_t s);
-hipError_t launch_store_levels(const SearchArgs& a, Level* out, hipStream_t s);
-// cell counts of every stride-th point (grid sizing estimate)
-hipError_t launch_hist_probe(const float* xyz, uint32_t n, uint32_t stride, const GridParams& gp, uint32_t* count,
-                             hipStream_t s);
-// coarser grid levels from the level-0 array (already sorted by a finer grid)
-hipError_t launch_hist_sorted(const float4* pts, uint32_t n, const GridParams& gp, uint32_t* count, uint32_t* cell,
-                              uint32_t* rank, hipStream_t s);
-hipError_t launch_scatter_sorted(const float4* in, uint32_t n, const uint32_t* cell, const uint32_t* rank,
-                                 const uint32_t* cell_start, float4* out, hipStream_t s);
 // reorder queries (and their bounds) by bound, largest first; count needs
-// 2049 entries, key and rank n
-hipError_t launch_order_by_bound(const float4* q, const float* bound, uint32_t n, uint32_t* count, uint32_t* key,
-                                 uint32_t* rank, uint32_t* scan_part, float4* q_out, float* bound_out, hipStream_t s);
-hipError_t launch_compact_queries(const float4* q, uint32_t n, const uint32_t* flag, const uint32_t* off,
-                                  float4* out, const float* bound, float* bound_out, hipStream_t s);
+// 2049 entries, key and rank n_max
+hipError_t launch_order_by_bound(const float4* q, const float* bound, uint32_t n_max, const uint64_t* n_dev,
+                                 uint32_t* count, uint32_t* key, uint32_t* rank, uint32_t* scan_part, float4* q_out,
+                                 float* bound_out, hipStream_t s);
+// dst[0..n) = host[0..n), the values passed as kernel arguments (no host
+// memory is read by the device: capturable into a graph)
+hipError_t launch_put_values(uint32_t* dst, const uint32_t* host, uint64_t n, hipStream_t s);
+// order[t] = original index of sorted point t
+hipError_t launch_order_from_sorted(const float4* pts, uint32_t n, uint32_t* order, hipStream_t s);
+// flag[n] = 0 for the device-side n (the exclusive scan's total slot)
+hipError_t launch_put_u32(uint32_t* p, const uint64_t* n_dev, uint32_t v, hipStream_t s);
+// compaction of the flagged queries; *out_count = off[n] (their number)
+hipError_t launch_compact_queries(const float4* q, uint32_t n_max, const uint64_t* n_dev, const uint32_t* flag,
+                                  const uint32_t* off, float4* out, const float* bound, float* bound_out,
+                                  uint64_t* out_count, hipStream_t s);
 hipError_t launch_arange_ptr(uint32_t* nn_ptr, uint32_t n_points, uint32_t k, hipStream_t s);
 hipError_t launch_compact_rows(const uint32_t* dense, uint32_t n_rows, uint32_t cap, const uint32_t* counts,
                                const uint32_t* row_ptr, uint32_t* nn, hipStream_t s);
@@ -144,7 +199,8 @@ enum SearchMode
     kModeKnn = 0,       // knn_search outputs
     kModeRadius = 1,    // radius_search outputs (dense, -1 padded)
     kModeKnnFeat = 2,   // fused: CSR row of neighbours + 11 features
-    kModeRadiusCsr = 3  // radius hits as u32 dense rows + counts (for CSR compaction)
+    kModeRadiusCsr = 3,   // radius hits as u32 dense rows + counts (for CSR compaction)
+    kModeRadiusMulti = 4  // kModeRadiusCsr + the multiscale features of every row (fused)
 };
 
 // ---- launchers (pgeof_features.hip) ----

@@ -104,9 +104,25 @@ __global__ __launch_bounds__(256) void k_bbox_final(const float* __restrict__ pa
 }
 
 // ---------------------------------------------------------- fill / ptr ----
-__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
+// number of entries a kernel works on: n_max, or min(*n_dev + add, n_max)
+// when the count is only known on the device
+__device__ __forceinline__ uint64_t dyn_count(uint64_t n_max, const uint64_t* n_dev, uint64_t add)
 {
+    if (!n_dev) return n_max;
+    const uint64_t v = *n_dev + add;
+    return v < n_max ? v : n_max;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n_max,
+                                                  const uint64_t* __restrict__ n_dev, uint64_t add, uint32_t v)
+{
+    const uint64_t n = dyn_count(n_max, n_dev, add);
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = v;
+}
+
+__global__ void k_put_u32(uint32_t* __restrict__ p, const uint64_t* __restrict__ idx, uint32_t v)
+{
+    if (threadIdx.x == 0) p[*idx] = v;
 }
 
 __global__ __launch_bounds__(256) void k_arange_ptr(uint32_t* __restrict__ p, uint32_t n_points, uint32_t k)
@@ -116,29 +132,87 @@ __global__ __launch_bounds__(256) void k_arange_ptr(uint32_t* __restrict__ p, ui
 }
 
 // ---------------------------------------------------------- histogram ----
-__global__ __launch_bounds__(256) void k_hist(const float* __restrict__ xyz, uint32_t n, GridParams gp,
+// Level l of the table (l < 0: the kNN sizing probe) and its number of cells
+// (0 when the level was not built).
+__device__ __forceinline__ Level level_of(const LevelTable* tab, int l) { return l < 0 ? tab->probe : tab->lv[l]; }
+__device__ __forceinline__ uint64_t cells_of(const LevelTable* tab, int l) { return l < 0 ? tab->probe_G : tab->G[l]; }
+
+__device__ __forceinline__ uint32_t cell_id(const Level& L, double x, double y, double z)
+{
+    const int cx = cell_coord(x, L.lo[0], L.inv_h, L.g[0]);
+    const int cy = cell_coord(y, L.lo[1], L.inv_h, L.g[1]);
+    const int cz = cell_coord(z, L.lo[2], L.inv_h, L.g[2]);
+    return ((uint32_t)cz * (uint32_t)L.g[1] + (uint32_t)cy) * (uint32_t)L.g[0] + (uint32_t)cx;
+}
+
+// Counting-sort histogram, 16 points per thread; rank = the point's rank in
+// its cell.  Grids of few cells count per block in LDS first (one global
+// atomic per touched cell and block: a few hot cells would otherwise
+// serialise the atomics), larger ones count in global memory directly.
+constexpr uint32_t kSmallHistCells = 8192;
+__global__ __launch_bounds__(256) void k_hist(const float* __restrict__ xyz, uint32_t n,
+                                              const LevelTable* __restrict__ tab, int l,
                                               uint32_t* __restrict__ count, uint32_t* __restrict__ cell,
                                               uint32_t* __restrict__ rank)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int cx = cell_coord((double)xyz[i * 3 + 0], gp.lo[0], gp.inv_h, gp.g[0]);
-    const int cy = cell_coord((double)xyz[i * 3 + 1], gp.lo[1], gp.inv_h, gp.g[1]);
-    const int cz = cell_coord((double)xyz[i * 3 + 2], gp.lo[2], gp.inv_h, gp.g[2]);
-    const uint32_t cid = ((uint32_t)cz * (uint32_t)gp.g[1] + (uint32_t)cy) * (uint32_t)gp.g[0] + (uint32_t)cx;
-    const uint32_t r = atomicAdd(&count[cid], 1u);
-    if (cell) cell[i] = cid;
-    if (rank) rank[i] = r;
+    __shared__ uint32_t bins[kSmallHistCells];
+    const uint64_t G = cells_of(tab, l);
+    if (G == 0) return;
+    const Level L = level_of(tab, l);
+    const bool small = G <= kSmallHistCells;
+    if (small)
+    {
+        for (uint32_t c = threadIdx.x; c < (uint32_t)G; c += 256) bins[c] = 0;
+        __syncthreads();
+    }
+    constexpr uint32_t kPer = 16;  // points per thread
+    const uint64_t base = (uint64_t)blockIdx.x * 256 * kPer;
+    uint32_t cid[kPer], lr[kPer];
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u)
+    {
+        const uint64_t i = base + (uint64_t)u * 256 + threadIdx.x;
+        cid[u] = 0xFFFFFFFFu;
+        lr[u] = 0;
+        if (i < n)
+        {
+            cid[u] = cell_id(L, (double)xyz[i * 3 + 0], (double)xyz[i * 3 + 1], (double)xyz[i * 3 + 2]);
+            lr[u] = atomicAdd(small ? &bins[cid[u]] : &count[cid[u]], 1u);
+        }
+    }
+    if (small)
+    {
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < (uint32_t)G; c += 256)
+        {
+            const uint32_t v = bins[c];
+            bins[c] = v ? atomicAdd(&count[c], v) : 0u;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kPer; ++u)
+    {
+        const uint64_t i = base + (uint64_t)u * 256 + threadIdx.x;
+        if (i < n)
+        {
+            cell[i] = cid[u];
+            rank[i] = small ? bins[cid[u]] + lr[u] : lr[u];
+        }
+    }
 }
 
 // Histogram of points that are already sorted by a finer grid (the level-0
 // array): neighbouring points mostly share a cell, so each run of equal cells
 // within a wave takes one atomic for the whole run.  rank = the run's base +
 // the lane's offset in the run; cell/rank are indexed by sorted position.
-__global__ __launch_bounds__(256) void k_hist_sorted(const float4* __restrict__ pts, uint32_t n, GridParams gp,
+__global__ __launch_bounds__(256) void k_hist_sorted(const float4* __restrict__ pts, uint32_t n,
+                                                     const LevelTable* __restrict__ tab, int l,
                                                      uint32_t* __restrict__ count, uint32_t* __restrict__ cell,
                                                      uint32_t* __restrict__ rank)
 {
+    if (cells_of(tab, l) == 0) return;
+    const Level L = level_of(tab, l);
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool active = i < n;
@@ -146,10 +220,7 @@ __global__ __launch_bounds__(256) void k_hist_sorted(const float4* __restrict__ 
     if (active)
     {
         const float4 p = pts[i];
-        const int cx = cell_coord((double)p.x, gp.lo[0], gp.inv_h, gp.g[0]);
-        const int cy = cell_coord((double)p.y, gp.lo[1], gp.inv_h, gp.g[1]);
-        const int cz = cell_coord((double)p.z, gp.lo[2], gp.inv_h, gp.g[2]);
-        cid = ((uint32_t)cz * (uint32_t)gp.g[1] + (uint32_t)cy) * (uint32_t)gp.g[0] + (uint32_t)cx;
+        cid = cell_id(L, (double)p.x, (double)p.y, (double)p.z);
     }
     const uint32_t prev = __shfl_up(cid, 1);
     const bool head = active && (lane == 0 || prev != cid);
@@ -175,28 +246,28 @@ __global__ __launch_bounds__(256) void k_scatter_sorted(const float4* __restrict
                                                         const uint32_t* __restrict__ cell,
                                                         const uint32_t* __restrict__ rank,
                                                         const uint32_t* __restrict__ cell_start,
-                                                        float4* __restrict__ out)
+                                                        float4* __restrict__ out, const LevelTable* __restrict__ tab,
+                                                        int l)
 {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || cells_of(tab, l) == 0) return;
     out[cell_start[cell[i]] + rank[i]] = in[i];
 }
 
-// Cell counts of every `stride`-th point (grid sizing estimate only).
+// Cell counts of every `stride`-th point at the probe grid (grid sizing estimate only).
 __global__ __launch_bounds__(256) void k_hist_probe(const float* __restrict__ xyz, uint32_t n, uint32_t stride,
-                                                    GridParams gp, uint32_t* __restrict__ count)
+                                                    const LevelTable* __restrict__ tab, uint32_t* __restrict__ count)
 {
     const uint64_t i = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * stride;
-    if (i >= n) return;
-    const int cx = cell_coord((double)xyz[i * 3 + 0], gp.lo[0], gp.inv_h, gp.g[0]);
-    const int cy = cell_coord((double)xyz[i * 3 + 1], gp.lo[1], gp.inv_h, gp.g[1]);
-    const int cz = cell_coord((double)xyz[i * 3 + 2], gp.lo[2], gp.inv_h, gp.g[2]);
-    atomicAdd(&count[((uint32_t)cz * (uint32_t)gp.g[1] + (uint32_t)cy) * (uint32_t)gp.g[0] + (uint32_t)cx], 1u);
+    if (i >= n || tab->probe_G == 0) return;
+    const Level L = tab->probe;
+    atomicAdd(&count[cell_id(L, (double)xyz[i * 3 + 0], (double)xyz[i * 3 + 1], (double)xyz[i * 3 + 2])], 1u);
 }
 
-__global__ __launch_bounds__(256) void k_sumsq(const uint32_t* __restrict__ count, uint64_t G,
-                                               unsigned long long* __restrict__ out)
+__global__ __launch_bounds__(256) void k_sumsq(const uint32_t* __restrict__ count, LevelTable* __restrict__ tab)
 {
+    const uint64_t G = tab->probe_G;
+    unsigned long long* out = &tab->sumsq;
     unsigned long long acc = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < G; i += (uint64_t)gridDim.x * 256)
     {
@@ -212,6 +283,158 @@ __global__ __launch_bounds__(256) void k_sumsq(const uint32_t* __restrict__ coun
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+
+// ------------------------------------------------------ grid planning ----
+// The grid of a search is sized on the device from the bounding box (and,
+// for kNN, a sampled occupancy probe), so that a search queues all its
+// launches without reading anything back: one thread computes the plan.
+
+__device__ uint64_t plan_cells_for(const double ext[3], double h, int g[3])
+{
+    uint64_t G = 1;
+    for (int a = 0; a < 3; ++a)
+    {
+        double c = floor(ext[a] / h) + 1.0;
+        if (!(c >= 1.0)) c = 1.0;
+        if (c > 1048576.0) c = 1048576.0;
+        g[a] = (int)c;
+        G *= (uint64_t)g[a];
+    }
+    return G;
+}
+
+// smallest h (up to 0.1 %) with plan_cells_for(h) <= cap
+__device__ double plan_h_for_cap(const double ext[3], uint64_t cap)
+{
+    const double emax = fmax(ext[0], fmax(ext[1], ext[2]));
+    if (!(emax > 0.0)) return 1.0;
+    int g[3];
+    double hi = emax * 2.0, lo = emax * 1e-7;
+    if (plan_cells_for(ext, lo, g) <= cap) return lo;
+    for (int it = 0; it < 80; ++it)
+    {
+        const double mid = sqrt(lo * hi);
+        if (plan_cells_for(ext, mid, g) <= cap)
+            hi = mid;
+        else
+            lo = mid;
+        if (hi / lo < 1.001) break;
+    }
+    return hi;
+}
+
+__device__ void plan_level(const float bb[6], double h, Level& L, uint64_t& G)
+{
+    double ext[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        L.lo[a] = (double)bb[a];
+        ext[a] = fmax(0.0, (double)bb[3 + a] - (double)bb[a]);
+    }
+    L.h = h;
+    L.inv_h = 1.0 / h;
+    L.cs = nullptr;
+    L.pts = nullptr;
+    G = plan_cells_for(ext, h, L.g);
+}
+
+__device__ void plan_extent(const float bb[6], double ext[3], double& emax)
+{
+    for (int a = 0; a < 3; ++a) ext[a] = fmax(0.0, (double)bb[3 + a] - (double)bb[a]);
+    emax = fmax(ext[0], fmax(ext[1], ext[2]));
+}
+
+// After the bounding box: the grid-size floor, and for kNN the probe grid of
+// the first guess of h (points per cell T, assuming a uniformly filled box).
+__global__ void k_plan_probe(PlanArgs p, LevelTable* __restrict__ tab)
+{
+    if (threadIdx.x != 0) return;
+    float bb[6];
+    bool finite = true;
+    for (int i = 0; i < 6; ++i) bb[i] = tab->bb[i];
+    for (int i = 0; i < 3; ++i)
+        if (!(bb[i] <= bb[3 + i])) finite = false;  // no finite point
+    if (!finite)
+        for (int i = 0; i < 6; ++i) bb[i] = 0.0f;
+    for (int i = 0; i < 6; ++i) tab->bb[i] = bb[i];
+    double ext[3], emax;
+    plan_extent(bb, ext, emax);
+    const double hcap = plan_h_for_cap(ext, p.cap);
+    tab->hcap = hcap;
+    tab->sumsq = 0;
+    tab->probe_G = 0;
+    tab->n_levels = 0;
+    for (int l = 0; l < kMaxLevels; ++l) tab->G[l] = 0;
+    if (!p.is_radius && emax > 0.0)
+    {
+        const double T = fmax(0.75, fmin(p.knn_occupancy * (double)p.k, p.knn_cell_points_max));
+        double vol = 1.0;
+        for (int a = 0; a < 3; ++a) vol *= fmax(ext[a], emax * 1e-3);
+        double h1 = cbrt(vol * T / fmax((double)p.n, 1.0));
+        h1 = fmax(h1, hcap);
+        Level L;
+        uint64_t G;
+        plan_level(bb, h1, L, G);
+        tab->probe = L;
+        tab->probe_G = G;
+        tab->T = T;
+        tab->h1 = h1;
+    }
+}
+
+// After the probe: level 0's cell edge (kNN: the first guess corrected by the
+// measured point-weighted occupancy, assuming points on 2-D surfaces; radius:
+// radius_cell_factor * r), then the pyramid levels that fit their tables.
+__global__ void k_plan_levels(PlanArgs p, LevelTable* __restrict__ tab)
+{
+    if (threadIdx.x != 0) return;
+    float bb[6];
+    for (int i = 0; i < 6; ++i) bb[i] = tab->bb[i];
+    double ext[3], emax;
+    plan_extent(bb, ext, emax);
+    const double hcap = tab->hcap;
+    double h;
+    if (!(emax > 0.0))
+        h = 1.0;
+    else if (p.is_radius)
+    {
+        const double r = fabs((double)p.radius);
+        h = isfinite(r) && r > 0.0 ? r * p.radius_cell_factor : emax;
+        h = fmax(h, hcap);
+    }
+    else
+    {
+        // sampling fraction pf: E[sum c_s^2] = pf^2 sum c^2 + pf (1 - pf) n
+        const uint32_t stride = p.probe_stride;
+        const double ns = (double)((p.n + stride - 1) / stride), pf = 1.0 / stride;
+        const double T = tab->T, h1 = tab->h1;
+        const double m1 = fmax(1.0, ((double)tab->sumsq - (1.0 - pf) * ns) / (pf * fmax(ns, 1.0)));
+        h = h1;
+        if (m1 > T) h = h1 * sqrt(T / m1);
+        h = fmax(h, hcap);
+    }
+    const double ratio = fmax(2.0, fmin(8.0, p.level_ratio));
+    int nl = 0;
+    for (int l = 0; l < p.max_levels; ++l)
+    {
+        Level L;
+        uint64_t G;
+        plan_level(bb, h * pow(ratio, (double)l), L, G);
+        if (l == 0 && G + 1 > p.cs_room[0]) plan_level(bb, hcap, L, G);  // (h >= hcap keeps G within the cap)
+        if (l > 0 && (G + 1 > p.cs_room[l] || tab->G[l - 1] <= 64)) break;
+        L.cs = p.cs[l];
+        L.pts = p.pts[l];
+        tab->lv[l] = L;
+        tab->G[l] = G;
+        nl = l + 1;
+    }
+    for (int l = nl; l < kMaxLevels; ++l)
+    {
+        tab->lv[l] = tab->lv[0];
+        tab->G[l] = 0;
+    }
+    tab->n_levels = nl;
 }
 
 // --------------------------------------------------------------- scan ----
@@ -255,24 +478,35 @@ __device__ __forceinline__ uint4 scan_load4(const uint32_t* in, uint64_t i, uint
     return v;
 }
 
+// The three scan kernels loop over the tiles (a block takes tiles b, b +
+// grid, ...), so that a launch sized for the largest input does no empty
+// work when the device-side count is smaller.
 template <bool VEC>
-__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n_max,
+                                                            const uint64_t* __restrict__ n_dev, uint64_t add,
                                                             uint32_t* __restrict__ partials)
 {
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    uint32_t s = 0;
-    for (int sub = 0; sub < kScanTile; sub += 4 * kScanBlock)
+    const uint64_t n = dyn_count(n_max, n_dev, add);
+    const uint64_t nt = (n + kScanTile - 1) / kScanTile;
+    for (uint64_t tile = blockIdx.x; tile < nt; tile += gridDim.x)
     {
-        const uint4 v = scan_load4<VEC>(in, base + sub + 4 * threadIdx.x, n);
-        s += v.x + v.y + v.z + v.w;
+        const uint64_t base = tile * kScanTile;
+        uint32_t s = 0;
+        for (int sub = 0; sub < kScanTile; sub += 4 * kScanBlock)
+        {
+            const uint4 v = scan_load4<VEC>(in, base + sub + 4 * threadIdx.x, n);
+            s += v.x + v.y + v.z + v.w;
+        }
+        uint32_t total;
+        block_exclusive_scan(s, total);
+        if (threadIdx.x == 0) partials[tile] = total;
     }
-    uint32_t total;
-    block_exclusive_scan(s, total);
-    if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t* __restrict__ partials, uint32_t nt)
+__global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t* __restrict__ partials, uint64_t n_max,
+                                                              const uint64_t* __restrict__ n_dev, uint64_t add)
 {
+    const uint32_t nt = (uint32_t)((dyn_count(n_max, n_dev, add) + kScanTile - 1) / kScanTile);
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nt; base += kScanTile)
     {
@@ -297,38 +531,45 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t* __restri
 }
 
 template <bool VEC>
-__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* in, uint32_t* out, uint64_t n,
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const uint32_t* in, uint32_t* out, uint64_t n_max,
+                                                           const uint64_t* __restrict__ n_dev, uint64_t add,
                                                            const uint32_t* __restrict__ partials)
 {
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    uint32_t carry = partials[blockIdx.x];
-    for (int sub = 0; sub < kScanTile; sub += 4 * kScanBlock)
+    const uint64_t n = dyn_count(n_max, n_dev, add);
+    const uint64_t nt = (n + kScanTile - 1) / kScanTile;
+    for (uint64_t tile = blockIdx.x; tile < nt; tile += gridDim.x)
     {
-        const uint64_t i = base + sub + 4 * threadIdx.x;
-        const uint4 v = scan_load4<VEC>(in, i, n);  // read before the block scan syncs: in may alias out
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v.x + v.y + v.z + v.w, total) + carry;
-        const uint4 o = make_uint4(ex, ex + v.x, ex + v.x + v.y, ex + v.x + v.y + v.z);
-        if (VEC && i + 3 < n)
-            reinterpret_cast<uint4*>(out)[i >> 2] = o;
-        else
+        const uint64_t base = tile * kScanTile;
+        uint32_t carry = partials[tile];
+        for (int sub = 0; sub < kScanTile; sub += 4 * kScanBlock)
         {
-            if (i < n) out[i] = o.x;
-            if (i + 1 < n) out[i + 1] = o.y;
-            if (i + 2 < n) out[i + 2] = o.z;
-            if (i + 3 < n) out[i + 3] = o.w;
+            const uint64_t i = base + sub + 4 * threadIdx.x;
+            const uint4 v = scan_load4<VEC>(in, i, n);  // read before the block scan syncs: in may alias out
+            uint32_t total;
+            const uint32_t ex = block_exclusive_scan(v.x + v.y + v.z + v.w, total) + carry;
+            const uint4 o = make_uint4(ex, ex + v.x, ex + v.x + v.y, ex + v.x + v.y + v.z);
+            if (VEC && i + 3 < n)
+                reinterpret_cast<uint4*>(out)[i >> 2] = o;
+            else
+            {
+                if (i < n) out[i] = o.x;
+                if (i + 1 < n) out[i + 1] = o.y;
+                if (i + 2 < n) out[i + 2] = o.z;
+                if (i + 3 < n) out[i + 3] = o.w;
+            }
+            carry += total;
         }
-        carry += total;
     }
 }
 
 // ------------------------------------------------------------ scatter ----
 __global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ xyz, uint32_t n,
                                                  const uint32_t* __restrict__ cell, const uint32_t* __restrict__ rank,
-                                                 const uint32_t* __restrict__ cell_start, float4* __restrict__ pts)
+                                                 const uint32_t* __restrict__ cell_start, float4* __restrict__ pts,
+                                                 const LevelTable* __restrict__ tab, int l)
 {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || cells_of(tab, l) == 0) return;
     const uint32_t pos = cell_start[cell[i]] + rank[i];
     pts[pos] = make_float4(xyz[i * 3 + 0], xyz[i * 3 + 1], xyz[i * 3 + 2], __uint_as_float((uint32_t)i));
 }
@@ -703,10 +944,27 @@ __device__ __forceinline__ int nth_near(int t, int c, int lo, int hi)
     return below > above ? c - r : c + r;
 }
 
-// Level l of the pyramid.  A level that differs between lanes is read from
-// the device copy of the table: indexing the kernel-argument copy with a
-// per-lane value would move the whole argument block to scratch memory.
-__device__ __forceinline__ Level pick_level(const SearchArgs& a, int l) { return a.levels[l]; }
+// Level l of the pyramid, read from the device-side table (a per-lane level
+// is a vector load into vector registers).
+__device__ __forceinline__ Level pick_level(const SearchArgs& a, int l) { return a.tab->lv[l]; }
+
+// A level every lane of the wave uses, moved to scalar registers: the table
+// is written by other kernels, so the compiler cannot load it with scalar
+// loads by itself, and the level's 18 dwords would otherwise occupy vector
+// registers for the whole search.
+__device__ __forceinline__ Level uniform_level(const LevelTable* tab, int l)
+{
+    static_assert(sizeof(Level) % 4 == 0, "Level is moved as dwords");
+    union
+    {
+        Level L;
+        uint32_t u[sizeof(Level) / 4];
+    } b;
+    b.L = tab->lv[l];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(Level) / 4); ++i) b.u[i] = __builtin_amdgcn_readfirstlane(b.u[i]);
+    return b.L;
+}
 
 #ifndef PGEOF_SCAN_UNROLL
 #define PGEOF_SCAN_UNROLL 4
@@ -718,16 +976,17 @@ constexpr int kHeapLds = 0;     // 4-ary, registers + LDS
 constexpr int kHeapGlobal = 1;  // binary, global memory (fallback)
 constexpr int kHeapTail = 2;    // 4-ary, registers + LDS + global tail past node 20
 
+// The search of query slot t (sorted position) by one lane; every lane of the
+// wave calls it together (inactive lanes take part in the wave-wide steps).
 template <int MODE, int WAVES, int HEAP, bool BUDGET>
-__global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
+__device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t t, const bool active,
+                                            unsigned char* smem, const uint32_t tid)
 {
-    constexpr bool RADIUS = (MODE == kModeRadius || MODE == kModeRadiusCsr);
+    constexpr bool RADIUS = (MODE == kModeRadius || MODE == kModeRadiusCsr || MODE == kModeRadiusMulti);
     constexpr int BLOCK = WAVES * 64;
-    extern __shared__ __align__(16) unsigned char smem[];
 
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint32_t t = xcd_logical_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    const bool active = t < a.n_query;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int n_levels = __builtin_amdgcn_readfirstlane(a.tab->n_levels);
     float4* stage = reinterpret_cast<float4*>(smem) + wid * kStage;
     unsigned char* hbase = smem + (size_t)WAVES * kStage * sizeof(float4);
 
@@ -744,12 +1003,13 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
     }
     else
     {
-        heap.keys = reinterpret_cast<uint64_t*>(hbase) + (size_t)threadIdx.x * 2;
+        heap.keys = reinterpret_cast<uint64_t*>(hbase) + (size_t)tid * 2;
         heap.pstride = 2 * BLOCK;
         if constexpr (HEAP == kHeapTail)
         {
-            // nodes past 20 in global memory (lane-interleaved pairs)
-            heap.gkeys = a.gheap_keys + (size_t)t * 2;
+            // nodes past 20 in global memory (lane-interleaved pairs, one
+            // slot per hardware lane of the launch)
+            heap.gkeys = a.gheap_keys + ((size_t)blockIdx.x * BLOCK + tid) * 2;
             heap.gpstride = (size_t)2 * gridDim.x * (size_t)BLOCK;
         }
     }
@@ -768,6 +1028,16 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
     // known upper bound of the k-th squared distance (from an earlier pass), or +inf
     const float qbound = (!RADIUS && a.query_bound && active) ? a.query_bound[t] : INFINITY;
     const bool bounded = qbound < INFINITY;
+    if (BUDGET && a.base_level >= n_levels)
+    {
+        // no such level: the query goes on to the next pass unchanged
+        if (active)
+        {
+            a.defer_flag[t] = 1u;
+            a.defer_bound[t] = qbound;
+        }
+        return;
+    }
     bool deferred = false;  // BUDGET: query handed over to the unbudgeted pass
     const float qxf = q4.x, qyf = q4.y, qzf = q4.z;
     const double qx = qxf, qy = qyf, qz = qzf;
@@ -777,11 +1047,11 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
     if (!RADIUS && !BUDGET && live && bounded)
     {
         const double rb = sqrt((double)qbound) * 1.00001;
-        while (level < a.n_levels - 1 && rb > 2.0 * pick_level(a, level).h) ++level;
+        while (level < n_levels - 1 && rb > 2.0 * pick_level(a, level).h) ++level;
     }
     else if (!RADIUS && !BUDGET && live)
     {
-        for (; level < a.n_levels - 1; ++level)
+        for (; level < n_levels - 1; ++level)
         {
             const Level L = pick_level(a, level);
             const int lx = cell_coord(qx, L.lo[0], L.inv_h, L.g[0]);
@@ -798,7 +1068,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
             if (count >= a.k) break;
         }
     }
-    const Level G = (RADIUS || BUDGET) ? a.lv[0] : pick_level(a, level);
+    const Level G = (RADIUS || BUDGET) ? uniform_level(a.tab, a.base_level) : pick_level(a, level);
     const float4* __restrict__ pts = G.pts;
     const uint32_t* __restrict__ cs = G.cs;
     const int G0 = G.g[0], G1 = G.g[1], G2 = G.g[2];
@@ -913,7 +1183,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
         {
             // every lane stages from the working level's array, also lanes
             // that sit on another level and have no row in this pass
-            const float4* __restrict__ wpts = (RADIUS || BUDGET) ? a.lv[0].pts : pick_level(a, lmin).pts;
+            const float4* __restrict__ wpts = (RADIUS || BUDGET) ? G.pts : pick_level(a, lmin).pts;
             // rows nearest to the wave's centre first: close candidates fill
             // the heaps early and most later ones fail the threshold test
             const int zc = a.near_first ? (Z0 + Z1) / 2 : Z0, yc = a.near_first ? (Y0 + Y1) / 2 : Y0;
@@ -1174,7 +1444,7 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
             if (od) od[j] = key == ~0ull ? INFINITY : __uint_as_float((uint32_t)(key >> 32));
         }
     }
-    else if (MODE == kModeRadius || MODE == kModeRadiusCsr)
+    else if (MODE == kModeRadius || MODE == kModeRadiusCsr || MODE == kModeRadiusMulti)
     {
         uint32_t cnt = 0;
         if (MODE == kModeRadius)
@@ -1209,6 +1479,51 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
             }
         }
         if (a.out_count) a.out_count[qorig] = cnt;
+        if (MODE == kModeRadiusMulti)
+        {
+            // compute_features_multiscale over this row (include/pgeof.hpp:
+            // 159-211): the features of its first k_s hits for every scale
+            // k_s <= cnt, in one walk of the sorted hits (the same shifted
+            // FP64 moments and eigensolve as the CSR kernel, so the values
+            // equal the two-call path's bit for bit); later scales stay 0.
+            const uint32_t S = a.n_scales;
+            float* of = a.out_feat + (size_t)qorig * S * 11;
+            uint32_t walk = 0;
+            for (uint32_t si = 0; si < S; ++si)
+            {
+                if (a.scales[si] > cnt) break;
+                walk = a.scales[si];
+            }
+            uint32_t si = 0;
+            if (walk)
+            {
+                // the hits are read back from the row just written (this
+                // lane's own stores): the heap is dead during the walk
+                const uint32_t* __restrict__ row = reinterpret_cast<const uint32_t*>(a.out_idx) + (size_t)qorig * k;
+                const float3* __restrict__ xyz3 = reinterpret_cast<const float3*>(a.data);
+                const float3 p0 = xyz3[row[0]];
+                const double ox = p0.x, oy = p0.y, oz = p0.z;
+                Moments m;
+                mom_zero(m);
+                for (uint32_t j = 0; j < walk; ++j)
+                {
+                    const float3 p = xyz3[row[j]];
+                    mom_add(m, (double)p.x - ox, (double)p.y - oy, (double)p.z - oz);
+                    while (si < S && a.scales[si] == j + 1)
+                    {
+                        double c[6];
+                        mom_cov(m, (double)(j + 1), c);
+                        const Eig E = eig_full(c);
+                        const Pca32 P = to_pca32(E);
+                        float f[11];
+                        features11(P, f);
+                        for (int cc = 0; cc < 11; ++cc) of[(size_t)si * 11 + cc] = f[cc];
+                        ++si;
+                    }
+                }
+            }
+            for (uint32_t c = si * 11; c < S * 11; ++c) of[c] = 0.0f;
+        }
     }
     else  // kModeKnnFeat
     {
@@ -1299,6 +1614,30 @@ __global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
         float* of = a.out_feat + (size_t)qorig * 11;
         for (int c = 0; c < 11; ++c) of[c] = f[c];
     }
+}
+
+// One lane per query.  With a device-side query count (the deferred pass)
+// the launch is sized for the largest possible count and the blocks past the
+// count exit.  Block b takes queries xcd_logical_block(b) * BLOCK ...:
+// consecutive (spatially adjacent) blocks of work share an XCD's L2.
+template <int MODE, int WAVES, int HEAP, bool BUDGET>
+__global__ __launch_bounds__(WAVES * 64, 4) void k_search(SearchArgs a)
+{
+    constexpr int BLOCK = WAVES * 64;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t nq = a.n_query, nb = gridDim.x;
+    if (a.n_query_dev)
+    {
+        const uint64_t c = *a.n_query_dev;
+        nq = c < (uint64_t)a.n_query ? (uint32_t)c : a.n_query;
+        nb = (nq + BLOCK - 1) / BLOCK;
+    }
+    // (a loop over several blocks of work per block would hold loop-invariant
+    // per-lane state through the whole search and spill: blocks past the
+    // work exit at once instead)
+    if (blockIdx.x >= nb) return;
+    const uint32_t t = xcd_logical_block(blockIdx.x, nb) * BLOCK + threadIdx.x;
+    search_slot<MODE, WAVES, HEAP, BUDGET>(a, t, t < nq, smem, threadIdx.x);
 }
 
 // ----------------------------------------------------------- selected ----
@@ -1556,6 +1895,9 @@ __global__ __launch_bounds__(256) void k_f64_to_f32(const double* __restrict__ i
         out[i] = (float)in[i];
 }
 
+// A search whose query count is only known on the device (n_query_dev) is
+// launched for the largest count, n_query: every query slot has a block, and
+// the blocks past the device-side count exit at once.
 template <int MODE, int WAVES, bool BUDGET>
 hipError_t launch_search_mb(const SearchArgs& a, size_t lds, bool gh, hipStream_t s)
 {
@@ -1583,62 +1925,24 @@ hipError_t launch_search_m(const SearchArgs& a, int block, size_t lds, bool gh, 
     }
 }
 
-__global__ __launch_bounds__(256) void k_compact_queries(const float4* __restrict__ q, uint32_t n,
+// The flagged queries (flag[i] = 1) of the first n, moved to off[i] (the
+// exclusive scan of the flags over n + 1 entries, flag[n] = 0); *out_count
+// receives off[n], their number.
+__global__ __launch_bounds__(256) void k_compact_queries(const float4* __restrict__ q, uint32_t n_max,
+                                                         const uint64_t* __restrict__ n_dev,
                                                          const uint32_t* __restrict__ flag,
                                                          const uint32_t* __restrict__ off, float4* __restrict__ out,
-                                                         const float* __restrict__ bound, float* __restrict__ bound_out)
+                                                         const float* __restrict__ bound, float* __restrict__ bound_out,
+                                                         uint64_t* __restrict__ out_count)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n && flag[i])
-    {
-        out[off[i]] = q[i];
-        if (bound) bound_out[off[i]] = bound[i];
-    }
-}
-
-// Histogram for grids with few cells: per-block counts in LDS, one global
-// atomic per touched cell and block; rank = block base + rank inside the block.
-constexpr uint32_t kSmallHistCells = 8192;
-__global__ __launch_bounds__(256) void k_hist_small(const float* __restrict__ xyz, uint32_t n, GridParams gp,
-                                                    uint32_t* __restrict__ count, uint32_t* __restrict__ cell,
-                                                    uint32_t* __restrict__ rank)
-{
-    __shared__ uint32_t bins[kSmallHistCells];
-    const uint32_t G = (uint32_t)gp.G;
-    for (uint32_t c = threadIdx.x; c < G; c += 256) bins[c] = 0;
-    __syncthreads();
-    constexpr uint32_t kPer = 16;  // points per thread
-    const uint64_t base = (uint64_t)blockIdx.x * 256 * kPer;
-    uint32_t cid[kPer], lr[kPer];
-    for (uint32_t u = 0; u < kPer; ++u)
-    {
-        const uint64_t i = base + (uint64_t)u * 256 + threadIdx.x;
-        cid[u] = 0xFFFFFFFFu;
-        if (i < n)
+    const uint64_t n = dyn_count(n_max, n_dev, 0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = off[n];
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u)
+        if (flag[i])
         {
-            const int cx = cell_coord((double)xyz[i * 3 + 0], gp.lo[0], gp.inv_h, gp.g[0]);
-            const int cy = cell_coord((double)xyz[i * 3 + 1], gp.lo[1], gp.inv_h, gp.g[1]);
-            const int cz = cell_coord((double)xyz[i * 3 + 2], gp.lo[2], gp.inv_h, gp.g[2]);
-            cid[u] = ((uint32_t)cz * (uint32_t)gp.g[1] + (uint32_t)cy) * (uint32_t)gp.g[0] + (uint32_t)cx;
-            lr[u] = atomicAdd(&bins[cid[u]], 1u);
+            out[off[i]] = q[i];
+            if (bound) bound_out[off[i]] = bound[i];
         }
-    }
-    __syncthreads();
-    for (uint32_t c = threadIdx.x; c < G; c += 256)
-    {
-        const uint32_t v = bins[c];
-        bins[c] = v ? atomicAdd(&count[c], v) : 0u;
-    }
-    __syncthreads();
-    for (uint32_t u = 0; u < kPer; ++u)
-    {
-        const uint64_t i = base + (uint64_t)u * 256 + threadIdx.x;
-        if (i < n)
-        {
-            if (cell) cell[i] = cid[u];
-            if (rank) rank[i] = bins[cid[u]] + lr[u];
-        }
-    }
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per, uint32_t cap)
@@ -1661,8 +1965,75 @@ hipError_t launch_bbox(const float* xyz, uint32_t n, float* partial, uint32_t n_
 
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
 {
+    return launch_fill_u32_dyn(p, n, nullptr, 0, v, s);
+}
+
+hipError_t launch_fill_u32_dyn(uint32_t* p, uint64_t n_max, const uint64_t* n_dev, uint64_t add, uint32_t v,
+                               hipStream_t s)
+{
+    if (n_max == 0) return hipSuccess;
+    k_fill_u32<<<blocks_for(n_max, 256, n_dev ? 4096 : 65536), 256, 0, s>>>(p, n_max, n_dev, add, v);
+    return hipGetLastError();
+}
+
+// Small host arrays travel to the device as kernel arguments (16 values per
+// launch), so that a call that uploads them can be captured into a graph.
+struct U32x16
+{
+    uint32_t v[16];
+};
+
+__global__ void k_put_values(uint32_t* __restrict__ dst, U32x16 vals, uint32_t count)
+{
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (uint32_t u = 0; u < 16; ++u)
+        if (u < count) dst[u] = vals.v[u];
+}
+
+// order[t] = original index of sorted point t (the spatial order of a self
+// search's rows)
+__global__ __launch_bounds__(256) void k_order_from_sorted(const float4* __restrict__ pts, uint32_t n,
+                                                           uint32_t* __restrict__ order)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) order[i] = __float_as_uint(pts[i].w);
+}
+
+hipError_t launch_put_values(uint32_t* dst, const uint32_t* host, uint64_t n, hipStream_t s)
+{
+    for (uint64_t b = 0; b < n; b += 16)
+    {
+        U32x16 v{};
+        const uint32_t c = (uint32_t)std::min<uint64_t>(16, n - b);
+        for (uint32_t u = 0; u < c; ++u) v.v[u] = host[b + u];
+        k_put_values<<<1, 64, 0, s>>>(dst + b, v, c);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_order_from_sorted(const float4* pts, uint32_t n, uint32_t* order, hipStream_t s)
+{
     if (n == 0) return hipSuccess;
-    k_fill_u32<<<blocks_for(n, 256, 65536), 256, 0, s>>>(p, n, v);
+    k_order_from_sorted<<<(n + 255) / 256, 256, 0, s>>>(pts, n, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_put_u32(uint32_t* p, const uint64_t* n_dev, uint32_t v, hipStream_t s)
+{
+    k_put_u32<<<1, 64, 0, s>>>(p, n_dev, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_probe(const PlanArgs& p, LevelTable* tab, hipStream_t s)
+{
+    k_plan_probe<<<1, 64, 0, s>>>(p, tab);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_levels(const PlanArgs& p, LevelTable* tab, hipStream_t s)
+{
+    k_plan_levels<<<1, 64, 0, s>>>(p, tab);
     return hipGetLastError();
 }
 
@@ -1672,15 +2043,12 @@ hipError_t launch_arange_ptr(uint32_t* nn_ptr, uint32_t n_points, uint32_t k, hi
     return hipGetLastError();
 }
 
-hipError_t launch_hist(const float* xyz, uint32_t n, const GridParams& gp, uint32_t* count, uint32_t* cell,
+hipError_t launch_hist(const float* xyz, uint32_t n, const LevelTable* tab, int l, uint32_t* count, uint32_t* cell,
                        uint32_t* rank, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     ProfScope ps("grid_histogram", s);
-    if (gp.G <= kSmallHistCells)
-        k_hist_small<<<(n + 256 * 16 - 1) / (256 * 16), 256, 0, s>>>(xyz, n, gp, count, cell, rank);
-    else
-        k_hist<<<(n + 255) / 256, 256, 0, s>>>(xyz, n, gp, count, cell, rank);
+    k_hist<<<(n + 256 * 16 - 1) / (256 * 16), 256, 0, s>>>(xyz, n, tab, l, count, cell, rank);
     return hipGetLastError();
 }
 
@@ -1698,10 +2066,13 @@ __device__ __forceinline__ uint32_t bound_bucket(float b)
     return (kBoundBuckets - 1) - min(bits, kBoundBuckets - 1);
 }
 
-__global__ __launch_bounds__(256) void k_bound_hist(const float* __restrict__ bound, uint32_t n,
+__global__ __launch_bounds__(256) void k_bound_hist(const float* __restrict__ bound, uint32_t n_max,
+                                                    const uint64_t* __restrict__ n_dev,
                                                     uint32_t* __restrict__ count, uint32_t* __restrict__ key,
                                                     uint32_t* __restrict__ rank)
 {
+    const uint64_t n = dyn_count(n_max, n_dev, 0);
+    if ((uint64_t)blockIdx.x * 256 * 16 >= n) return;
     __shared__ uint32_t bins[kBoundBuckets];
     for (uint32_t c = threadIdx.x; c < kBoundBuckets; c += 256) bins[c] = 0;
     __syncthreads();
@@ -1740,59 +2111,40 @@ __global__ __launch_bounds__(256) void k_bound_hist(const float* __restrict__ bo
 }
 
 __global__ __launch_bounds__(256) void k_bound_scatter(const float4* __restrict__ q, const float* __restrict__ bound,
-                                                       uint32_t n, const uint32_t* __restrict__ key,
+                                                       uint32_t n_max, const uint64_t* __restrict__ n_dev,
+                                                       const uint32_t* __restrict__ key,
                                                        const uint32_t* __restrict__ rank,
                                                        const uint32_t* __restrict__ start, float4* __restrict__ q_out,
                                                        float* __restrict__ bound_out)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
+    if (i >= dyn_count(n_max, n_dev, 0)) return;
     const uint32_t p = start[key[i]] + rank[i];
     q_out[p] = q[i];
     bound_out[p] = bound[i];
 }
 
-hipError_t launch_order_by_bound(const float4* q, const float* bound, uint32_t n, uint32_t* count, uint32_t* key,
-                                 uint32_t* rank, uint32_t* scan_part, float4* q_out, float* bound_out, hipStream_t s)
+hipError_t launch_order_by_bound(const float4* q, const float* bound, uint32_t n_max, const uint64_t* n_dev,
+                                 uint32_t* count, uint32_t* key, uint32_t* rank, uint32_t* scan_part, float4* q_out,
+                                 float* bound_out, hipStream_t s)
 {
-    if (n == 0) return hipSuccess;
+    if (n_max == 0) return hipSuccess;
     ProfScope ps("knn_defer_order", s);
     hipError_t e = launch_fill_u32(count, kBoundBuckets + 1, 0u, s);
     if (e != hipSuccess) return e;
-    k_bound_hist<<<(n + 256 * 16 - 1) / (256 * 16), 256, 0, s>>>(bound, n, count, key, rank);
+    k_bound_hist<<<(n_max + 256 * 16 - 1) / (256 * 16), 256, 0, s>>>(bound, n_max, n_dev, count, key, rank);
     e = launch_scan_exclusive(count, count, kBoundBuckets + 1, scan_part, s);
     if (e != hipSuccess) return e;
-    k_bound_scatter<<<(n + 255) / 256, 256, 0, s>>>(q, bound, n, key, rank, count, q_out, bound_out);
+    k_bound_scatter<<<(n_max + 255) / 256, 256, 0, s>>>(q, bound, n_max, n_dev, key, rank, count, q_out, bound_out);
     return hipGetLastError();
 }
 
-__global__ void k_store_levels(SearchArgs a, Level* out)
+hipError_t launch_compact_queries(const float4* q, uint32_t n_max, const uint64_t* n_dev, const uint32_t* flag,
+                                  const uint32_t* off, float4* out, const float* bound, float* bound_out,
+                                  uint64_t* out_count, hipStream_t s)
 {
-    if (threadIdx.x != 0) return;
-    out[0] = a.lv[0];
-    out[1] = a.lv[1];
-    out[2] = a.lv[2];
-    out[3] = a.lv[3];
-}
-
-hipError_t launch_store_levels(const SearchArgs& a, Level* out, hipStream_t s)
-{
-    static_assert(kMaxLevels == 4, "k_store_levels copies four levels");
-    k_store_levels<<<1, 64, 0, s>>>(a, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_compact_queries(const float4* q, uint32_t n, const uint32_t* flag, const uint32_t* off,
-                                  float4* out, const float* bound, float* bound_out, hipStream_t s)
-{
-    if (n == 0) return hipSuccess;
-    k_compact_queries<<<(n + 255) / 256, 256, 0, s>>>(q, n, flag, off, out, bound, bound_out);
-    return hipGetLastError();
-}
-
-hipError_t launch_sumsq(const uint32_t* count, uint64_t G, unsigned long long* out, hipStream_t s)
-{
-    k_sumsq<<<blocks_for(G, 256, 2048), 256, 0, s>>>(count, G, out);
+    k_compact_queries<<<blocks_for(n_max, 256, 16384), 256, 0, s>>>(q, n_max, n_dev, flag, off, out, bound,
+                                                                     bound_out, out_count);
     return hipGetLastError();
 }
 
@@ -1800,56 +2152,67 @@ size_t scan_partials_needed(uint64_t n) { return (size_t)((n + kScanTile - 1) / 
 
 hipError_t launch_scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partials, hipStream_t s)
 {
-    if (n == 0) return hipSuccess;
+    return launch_scan_exclusive_dyn(in, out, n, nullptr, 0, partials, s);
+}
+
+hipError_t launch_scan_exclusive_dyn(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint64_t* n_dev,
+                                     uint64_t add, uint32_t* partials, hipStream_t s)
+{
+    if (n_max == 0) return hipSuccess;
     ProfScope ps("grid_scan", s);
-    const uint32_t nt = (uint32_t)((n + kScanTile - 1) / kScanTile);
+    const uint64_t nt = (n_max + kScanTile - 1) / kScanTile;
+    const uint32_t nb = (uint32_t)std::min<uint64_t>(nt, 2048);
     const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
     if (vec)
-        k_scan_reduce<true><<<nt, kScanBlock, 0, s>>>(in, n, partials);
+        k_scan_reduce<true><<<nb, kScanBlock, 0, s>>>(in, n_max, n_dev, add, partials);
     else
-        k_scan_reduce<false><<<nt, kScanBlock, 0, s>>>(in, n, partials);
-    k_scan_partials<<<1, kScanBlock, 0, s>>>(partials, nt);
+        k_scan_reduce<false><<<nb, kScanBlock, 0, s>>>(in, n_max, n_dev, add, partials);
+    k_scan_partials<<<1, kScanBlock, 0, s>>>(partials, n_max, n_dev, add);
     if (vec)
-        k_scan_apply<true><<<nt, kScanBlock, 0, s>>>(in, out, n, partials);
+        k_scan_apply<true><<<nb, kScanBlock, 0, s>>>(in, out, n_max, n_dev, add, partials);
     else
-        k_scan_apply<false><<<nt, kScanBlock, 0, s>>>(in, out, n, partials);
+        k_scan_apply<false><<<nb, kScanBlock, 0, s>>>(in, out, n_max, n_dev, add, partials);
     return hipGetLastError();
 }
 
-hipError_t launch_hist_probe(const float* xyz, uint32_t n, uint32_t stride, const GridParams& gp, uint32_t* count,
-                             hipStream_t s)
+hipError_t launch_probe(const float* xyz, uint32_t n, uint32_t stride, LevelTable* tab, uint32_t* count,
+                        uint64_t count_room, hipStream_t s)
 {
     const uint32_t ns = (n + stride - 1) / stride;
     if (ns == 0) return hipSuccess;
     ProfScope ps("grid_probe", s);
-    k_hist_probe<<<(ns + 255) / 256, 256, 0, s>>>(xyz, n, stride, gp, count);
+    hipError_t e = launch_fill_u32_dyn(count, count_room, &tab->probe_G, 0, 0u, s);
+    if (e != hipSuccess) return e;
+    k_hist_probe<<<(ns + 255) / 256, 256, 0, s>>>(xyz, n, stride, tab, count);
+    k_sumsq<<<blocks_for(count_room, 256, 2048), 256, 0, s>>>(count, tab);
     return hipGetLastError();
 }
 
-hipError_t launch_hist_sorted(const float4* pts, uint32_t n, const GridParams& gp, uint32_t* count, uint32_t* cell,
-                              uint32_t* rank, hipStream_t s)
+hipError_t launch_hist_sorted(const float4* pts, uint32_t n, const LevelTable* tab, int l, uint32_t* count,
+                              uint32_t* cell, uint32_t* rank, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     ProfScope ps("grid_histogram_sorted", s);
-    k_hist_sorted<<<(n + 255) / 256, 256, 0, s>>>(pts, n, gp, count, cell, rank);
+    k_hist_sorted<<<(n + 255) / 256, 256, 0, s>>>(pts, n, tab, l, count, cell, rank);
     return hipGetLastError();
 }
 
 hipError_t launch_scatter_sorted(const float4* in, uint32_t n, const uint32_t* cell, const uint32_t* rank,
-                                 const uint32_t* cell_start, float4* out, hipStream_t s)
+                                 const uint32_t* cell_start, float4* out, const LevelTable* tab, int l,
+                                 hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     ProfScope ps("grid_scatter", s);
-    k_scatter_sorted<<<(n + 255) / 256, 256, 0, s>>>(in, n, cell, rank, cell_start, out);
+    k_scatter_sorted<<<(n + 255) / 256, 256, 0, s>>>(in, n, cell, rank, cell_start, out, tab, l);
     return hipGetLastError();
 }
 
 hipError_t launch_scatter(const float* xyz, uint32_t n, const uint32_t* cell, const uint32_t* rank,
-                          const uint32_t* cell_start, float4* pts, hipStream_t s)
+                          const uint32_t* cell_start, float4* pts, const LevelTable* tab, int l, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     ProfScope ps("grid_scatter", s);
-    k_scatter<<<(n + 255) / 256, 256, 0, s>>>(xyz, n, cell, rank, cell_start, pts);
+    k_scatter<<<(n + 255) / 256, 256, 0, s>>>(xyz, n, cell, rank, cell_start, pts, tab, l);
     return hipGetLastError();
 }
 
@@ -1878,10 +2241,12 @@ hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream
 
 hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds, bool gh, bool budget, hipStream_t s)
 {
-    static const char* names[4] = {"knn_search", "radius_search", "knn_features_fused", "radius_search_csr"};
-    static const char* names_b[4] = {"knn_search_dense", "radius_search", "knn_features_fused_dense",
-                                     "radius_search_csr"};
-    ProfScope ps(budget ? names_b[mode & 3] : names[mode & 3], s);
+    static const char* names[5] = {"knn_search", "radius_search", "knn_features_fused", "radius_search_csr",
+                                   "radius_multiscale_fused"};
+    static const char* names_b[5] = {"knn_search_dense", "radius_search", "knn_features_fused_dense",
+                                     "radius_search_csr", "radius_multiscale_fused"};
+    if (mode < 0 || mode > 4) return hipErrorInvalidValue;
+    ProfScope ps(budget ? names_b[mode] : names[mode], s);
     if (budget)
     {
         switch (mode)
@@ -1896,6 +2261,7 @@ hipError_t launch_search(int mode, const SearchArgs& a, int block, size_t lds, b
         case kModeKnn: return launch_search_m<kModeKnn, false>(a, block, lds, gh, s);
         case kModeRadius: return launch_search_m<kModeRadius, false>(a, block, lds, gh, s);
         case kModeKnnFeat: return launch_search_m<kModeKnnFeat, false>(a, block, lds, gh, s);
+        case kModeRadiusMulti: return launch_search_m<kModeRadiusMulti, false>(a, block, lds, gh, s);
         default: return launch_search_m<kModeRadiusCsr, false>(a, block, lds, gh, s);
     }
 }
