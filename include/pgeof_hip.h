@@ -177,6 +177,25 @@ int pgeof_compute_features_optimal_device(const float* xyz, int64_t n_xyz, const
                                           uint32_t k_step, uint32_t k_min_search, float* features, uint32_t* err,
                                           void* stream);
 
+/* Largest CSR (rows = n_nn_ptr - 1) that pgeof_compute_features_device_noalloc
+ * accepts. */
+#define PGEOF_NOALLOC_MAX_ROWS 4194304
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pgeof_compute_features_device with its scratch (the
+ * spatial row order of inputs of 65536 rows or more) taken from a 60 MiB
+ * array that the library holds on each device it runs on until the process
+ * ends.  It calls no allocator and never synchronises: err (a device u32,
+ * zeroed by the caller) is required and receives the error bits.  At most
+ * PGEOF_NOALLOC_MAX_ROWS rows (EINVAL beyond).  Calls on one device share
+ * the array: calls that overlap in time (different streams, no ordering
+ * between them) race; order them on one stream.  Results equal those of
+ * pgeof_compute_features_device bit for bit. */
+int pgeof_compute_features_device_noalloc(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                          const uint32_t* nn_ptr, int64_t n_nn_ptr, uint64_t k_min,
+                                          float* features, uint32_t* err, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Serving loop (no counterpart in the reference; the reference processes one  */
 /* cloud per call on the CPU).  Requests are point clouds of any size; each    */

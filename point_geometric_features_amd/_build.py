@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("PGEOF_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pgeof_search.hip", "pgeof_features.hip", "pgeof_capi.cpp", "pgeof_serve.cpp"]
+HIP_SOURCES = ["pgeof_search.hip", "pgeof_features.hip", "pgeof_noalloc.hip", "pgeof_capi.cpp", "pgeof_serve.cpp"]
 
 # packed FP32 instructions are kept out of the device code on purpose
 HIP_FLAGS = [

@@ -51,6 +51,7 @@ _SIGS = {
     "pgeof_radius_multiscale_workspace_size": (_SZ, [_I64, _U32, _I64]),
     "pgeof_radius_multiscale_device": (ctypes.c_int, [_P, _I64, _F, _U32, _P, _I64, _P, _P, _P, _P, _SZ, _P]),
     "pgeof_knn_optimal_device": (ctypes.c_int, [_P, _I64, _U32, _U32, _U32, _U32, _P, _P, _P, _P, _SZ, _P]),
+    "pgeof_compute_features_device_noalloc": (ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _U64, _P, _P, _P]),
     "pgeof_compute_features_multiscale_device": (ctypes.c_int,
                                                  [_P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P]),
     "pgeof_compute_features_optimal_device": (ctypes.c_int,
@@ -222,6 +223,17 @@ def compute_features_device(xyz, nn, nn_ptr, features, k_min=1, err=None, stream
     check(lib().pgeof_compute_features_device(_ptr(xyz), xyz.shape[0], _ptr(nn), nn.shape[0], _ptr(nn_ptr),
                                               nn_ptr.shape[0], int(k_min), _ptr(features), _ptr(err),
                                               _stream(stream)))
+
+
+def compute_features_device_noalloc(xyz, nn, nn_ptr, features, err, k_min=1, stream=None):
+    """pgeof_compute_features_device_noalloc: no allocation, no synchronisation;
+    `err` (one int32/uint32 device word, zeroed by the caller) is required."""
+    check(lib().pgeof_compute_features_device_noalloc(_ptr(xyz), xyz.shape[0], _ptr(nn), nn.shape[0], _ptr(nn_ptr),
+                                                      nn_ptr.shape[0], int(k_min), _ptr(features), _ptr(err),
+                                                      _stream(stream)))
+
+
+NOALLOC_MAX_ROWS = 4194304  # PGEOF_NOALLOC_MAX_ROWS
 
 
 def compute_features_multiscale_device(xyz, nn, nn_ptr, k_scales, features, err=None, stream=None):

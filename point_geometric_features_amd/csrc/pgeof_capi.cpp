@@ -1392,6 +1392,30 @@ int pgeof_compute_features_device(const float* xyz, int64_t n_xyz, const uint32_
                               stream);
 }
 
+int pgeof_compute_features_device_noalloc(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
+                                          const uint32_t* nn_ptr, int64_t n_nn_ptr, uint64_t k_min, float* features,
+                                          uint32_t* err, void* stream)
+{
+    TuningSnapshot snapshot;
+    if (k_min < 1) return fail(PGEOF_EINVAL, "k_min should be > 1");
+    int rc = validate_csr(n_xyz, n_nn, n_nn_ptr);
+    if (rc) return rc;
+    if (!err) return fail(PGEOF_EINVAL, "err must be a device word: the no-allocation entry point never synchronises");
+    if ((uint64_t)(n_nn_ptr - 1) > PGEOF_NOALLOC_MAX_ROWS)
+        return fail(PGEOF_EINVAL, "more rows than the no-allocation entry point supports (PGEOF_NOALLOC_MAX_ROWS = " +
+                                      std::to_string(PGEOF_NOALLOC_MAX_ROWS) + ")");
+    if (n_nn_ptr - 1 == 0) return PGEOF_OK;
+    // the library's static scratch on this device (row-order buffers); it
+    // holds no state between calls, everything in it is rewritten per call
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    hipError_t e = noalloc_scratch(&ws, &ws_bytes);
+    if (e != hipSuccess) return hip_fail(e, "no-allocation scratch");
+    const uint32_t km = (uint32_t)std::min<uint64_t>(k_min, 0xFFFFFFFFull);
+    return csr_device(0, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, km, 1, 1, nullptr, 0, features, err,
+                      (hipStream_t)stream, ws, ws_bytes);
+}
+
 int pgeof_compute_features_multiscale_device(const float* xyz, int64_t n_xyz, const uint32_t* nn, int64_t n_nn,
                                              const uint32_t* nn_ptr, int64_t n_nn_ptr,
                                              const uint32_t* k_scales_host, int64_t n_scales, float* features,

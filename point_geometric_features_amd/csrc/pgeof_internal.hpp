@@ -226,6 +226,11 @@ hipError_t launch_csr_features(int mode, const CsrArgs& a, hipStream_t s);
 size_t csr_order_bytes(uint32_t n_points);
 hipError_t launch_csr_order(const CsrArgs& a, void* ws, uint32_t* order[1], hipStream_t s);
 
+// ---- static scratch of the no-allocation entry point (pgeof_noalloc.hip) ----
+// Device address and size of this library's file-scope scratch array on the
+// current device, looked up once per device.
+hipError_t noalloc_scratch(void** ptr, size_t* bytes);
+
 // ---- profiling hooks (pgeof_capi.cpp) ----
 void prof_begin(const char* name, hipStream_t s, void** token);
 void prof_end(void* token, hipStream_t s);
