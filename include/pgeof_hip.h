@@ -94,8 +94,9 @@ int pgeof_compute_features_selected_f64(const double* xyz, int64_t n, double sea
 /* ------------------------------------------------------------------------ */
 /* Device-memory entry points.  All pointers are device pointers on the       */
 /* current HIP device; work is ordered on `stream` (a hipStream_t, NULL = the */
-/* null stream).  `workspace` may be NULL: the call then takes and returns    */
-/* its scratch with hipMallocAsync/hipFreeAsync on `stream`.  The searches    */
+/* null stream).  `workspace` may be NULL: the call then borrows a buffer of  */
+/* the library's scratch cache (allocated once, reused once the stream that   */
+/* last used it has passed an event recorded after its launches).  The searches */
 /* size their grids on the device and never synchronise `stream`: given a     */
 /* workspace (and, for pgeof_radius_csr_device, nnz_out = NULL) a call only   */
 /* queues kernels, so it can be captured into a HIP graph.                    */

@@ -5,6 +5,7 @@ re-exports its extension module's names.  This package does the same over the
 gfx950 implementation, so ``import pgeof`` works unchanged for code written
 against the reference.
 """
+import point_geometric_features_amd as _impl
 from point_geometric_features_amd import (  # noqa: F401
     EFeatureID,
     compute_features,
@@ -14,7 +15,8 @@ from point_geometric_features_amd import (  # noqa: F401
     knn_search,
     radius_search,
 )
-from point_geometric_features_amd.pgeof_ext import *  # noqa: F401,F403  (EFeatureID values)
+
+globals().update({_k: _v for _k, _v in EFeatureID.__members__.items()})  # export_values()
 
 __all__ = [
     "EFeatureID",

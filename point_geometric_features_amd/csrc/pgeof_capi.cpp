@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -204,6 +205,99 @@ int lds_limit()
     }
     return d.lds_max;
 }
+
+// Scratch of device calls made without a workspace: device buffers kept by
+// the library (hipMalloc, per device), each handed out to one call at a time
+// and handed out again once an event recorded on that call's stream after
+// its last launch has completed.  Stream-ordered allocation
+// (hipMallocAsync / hipFreeAsync) is not used: the buffers stay mapped for
+// the life of the process.
+struct ScratchBuf
+{
+    int dev;
+    void* p;
+    size_t bytes;
+    hipEvent_t done;  // recorded on the last user's stream when it returned the buffer
+    bool busy;
+    hipStream_t last;  // that stream (a call on the same stream may reuse the buffer at once)
+};
+std::mutex g_scratch_mu;
+std::vector<std::unique_ptr<ScratchBuf>> g_scratch;  // stable addresses: leases hold pointers
+
+// A buffer of at least `need` bytes on the current device, not in use by any
+// call still running.  Free buffers too small for the request are released
+// first (hipFree), so the cache holds about one buffer per concurrent call.
+int scratch_acquire(size_t need, hipStream_t s, ScratchBuf** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    need = (need + 4095) & ~size_t(4095);
+    std::lock_guard<std::mutex> l(g_scratch_mu);
+    ScratchBuf* best = nullptr;
+    for (auto& up : g_scratch)
+    {
+        ScratchBuf* b = up.get();
+        if (b->dev != dev || b->busy || b->bytes < need) continue;
+        if (b->last != s && hipEventQuery(b->done) != hipSuccess) continue;  // still in use on another stream
+        if (!best || b->bytes < best->bytes) best = b;
+    }
+    if (!best)
+    {
+        for (size_t i = 0; i < g_scratch.size();)
+        {
+            ScratchBuf* b = g_scratch[i].get();
+            if (b->dev == dev && !b->busy && b->bytes < need && hipEventQuery(b->done) == hipSuccess)
+            {
+                (void)hipFree(b->p);
+                (void)hipEventDestroy(b->done);
+                g_scratch.erase(g_scratch.begin() + (ptrdiff_t)i);
+            }
+            else
+                ++i;
+        }
+        auto b = std::make_unique<ScratchBuf>(ScratchBuf{dev, nullptr, need, nullptr, false, s});
+        HIP_TRY(hipMalloc(&b->p, need));
+        hipError_t e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+        if (e != hipSuccess)
+        {
+            (void)hipFree(b->p);
+            return hip_fail(e, "hipEventCreateWithFlags");
+        }
+        best = b.get();
+        g_scratch.push_back(std::move(b));
+    }
+    best->busy = true;
+    *out = best;
+    return PGEOF_OK;
+}
+
+// the caller's launches on s are queued: the buffer is free once s gets here
+void scratch_release(ScratchBuf* b, hipStream_t s)
+{
+    std::lock_guard<std::mutex> l(g_scratch_mu);
+    (void)hipEventRecord(b->done, s);
+    b->last = s;
+    b->busy = false;
+}
+
+struct ScratchLease
+{
+    ScratchBuf* buf = nullptr;
+    hipStream_t s = nullptr;
+    void* p = nullptr;
+    int take(size_t need, hipStream_t st)
+    {
+        const int rc = scratch_acquire(need, st, &buf);
+        if (rc) return rc;
+        s = st;
+        p = buf->p;
+        return PGEOF_OK;
+    }
+    ~ScratchLease()
+    {
+        if (buf) scratch_release(buf, s);
+    }
+};
 
 // Per-thread host-API context: a stream, a grow-only device buffer and a
 // small pinned host buffer for read-backs.
@@ -720,7 +814,7 @@ int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, in
     // plain, 10 M rows of 20: 3.9 -> 2.6 ms plus 0.8 ms for the order.  The
     // multiscale kernel gains what the order costs (2 M rows: 2.0 -> 1.8 +
     // 0.2 ms) and keeps index order.
-    void* tmp = nullptr;
+    ScratchLease tmp;
     if (mode != 1 && a.n_points >= 65536 && n_xyz > 0 && tuning().csr_order > 0.5)
     {
         void* ows = order_ws;
@@ -730,20 +824,16 @@ int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, in
         }
         else
         {
-            HIP_TRY(hipMallocAsync(&tmp, csr_order_bytes(a.n_points), s));
-            ows = tmp;
+            const int rc = tmp.take(csr_order_bytes(a.n_points), s);
+            if (rc) return rc;
+            ows = tmp.p;
         }
         uint32_t* ord[1] = {nullptr};
         hipError_t e = launch_csr_order(a, ows, ord, s);
-        if (e != hipSuccess)
-        {
-            if (tmp) (void)hipFreeAsync(tmp, s);
-            return hip_fail(e, "launch_csr_order");
-        }
+        if (e != hipSuccess) return hip_fail(e, "launch_csr_order");
         a.order = ord[0];
     }
     hipError_t e = launch_csr_features(mode, a, s);
-    if (tmp) (void)hipFreeAsync(tmp, s);
     if (e != hipSuccess) return hip_fail(e, "launch_csr_features");
     return PGEOF_OK;
 }
@@ -786,29 +876,21 @@ int csr_host(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, int6
     return err_from_word(werr);
 }
 
-// Device-memory wrapper: workspace from the caller or stream-ordered.
-struct WsGuard
-{
-    void* p = nullptr;
-    hipStream_t s;
-    bool owned = false;
-    ~WsGuard()
-    {
-        if (owned && p) (void)hipFreeAsync(p, s);
-    }
-};
+// Device-memory wrapper: workspace from the caller or from the library's
+// scratch cache (returned when the entry point returns, reusable once the
+// stream has passed its last launch).
+using WsGuard = ScratchLease;
 
 int acquire_ws(WsGuard& g, void* ws, size_t ws_bytes, size_t need, hipStream_t s, Arena& ar)
 {
-    g.s = s;
     if (ws)
     {
         if (ws_bytes < need) return fail(PGEOF_EINVAL, "workspace smaller than the size returned by *_workspace_size");
         ar = Arena{(char*)ws, ws_bytes, 0};
         return PGEOF_OK;
     }
-    HIP_TRY(hipMallocAsync(&g.p, need, s));
-    g.owned = true;
+    const int rc = g.take(need, s);
+    if (rc) return rc;
     ar = Arena{(char*)g.p, need, 0};
     return PGEOF_OK;
 }
@@ -1348,34 +1430,28 @@ static int csr_device_checked(int mode, const float* xyz, int64_t n_xyz, const u
 {
     hipStream_t s = (hipStream_t)stream;
     if (n_nn_ptr - 1 == 0) return PGEOF_OK;
-    uint32_t* d_sc = nullptr;
     uint32_t* d_err = err;
-    void* tmp = nullptr;
-    const size_t need = al((size_t)n_scales * 4 + 4) + al(4);
-    HIP_TRY(hipMallocAsync(&tmp, need, s));
-    d_sc = (uint32_t*)tmp;
-    if (n_scales) HIP_TRY(hipMemcpyAsync(d_sc, scales_host, (size_t)n_scales * 4, hipMemcpyHostToDevice, s));
+    ScratchLease tmp;
+    int rc = tmp.take(al((size_t)n_scales * 4 + 4) + al(4), s);
+    if (rc) return rc;
+    uint32_t* d_sc = (uint32_t*)tmp.p;
+    // the scales travel as kernel arguments (no copy from host memory)
+    if (n_scales) HIP_TRY(launch_put_values(d_sc, scales_host, (uint64_t)n_scales, s));
     if (!d_err)
     {
-        d_err = (uint32_t*)((char*)tmp + al((size_t)n_scales * 4 + 4));
+        d_err = (uint32_t*)((char*)tmp.p + al((size_t)n_scales * 4 + 4));
         HIP_TRY(launch_fill_u32(d_err, 1, 0u, s));
     }
-    int rc = csr_device(mode, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, k_min, k_step, k_min_search, d_sc,
-                        (uint32_t)n_scales, out, d_err, s);
-    if (rc)
-    {
-        (void)hipFreeAsync(tmp, s);
-        return rc;
-    }
+    rc = csr_device(mode, xyz, n_xyz, nn, n_nn, nn_ptr, n_nn_ptr, k_min, k_step, k_min_search, d_sc,
+                    (uint32_t)n_scales, out, d_err, s);
+    if (rc) return rc;
     if (!err)
     {
         uint32_t w = 0;
         rc = readback(&w, d_err, 4, s);
-        (void)hipFreeAsync(tmp, s);
         if (rc) return rc;
         return err_from_word(w);
     }
-    HIP_TRY(hipFreeAsync(tmp, s));
     return PGEOF_OK;
 }
 

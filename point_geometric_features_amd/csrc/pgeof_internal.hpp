@@ -108,7 +108,12 @@ struct SearchArgs
     uint32_t n_scales;
 };
 
-// Simple linear device arena carved out of one allocation.
+// Simple linear device arena carved out of one allocation.  Every region
+// starts on a 256-byte boundary of the address space, not only of the
+// offset: regions hold 64-bit atomics and 16-byte vectors, and the base of an
+// arena (a caller's workspace, or the rest of a buffer after outputs of odd
+// length) may be aligned to 4 bytes only.  A misaligned base costs at most
+// 255 bytes, inside the 4096 bytes of slack every workspace size adds.
 struct Arena
 {
     char* base = nullptr;
@@ -117,7 +122,8 @@ struct Arena
     template <typename T>
     T* take(size_t count)
     {
-        size_t a = (off + 255) & ~size_t(255);
+        const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+        const size_t a = (size_t)(((b + off + 255) & ~uintptr_t(255)) - b);
         off = a + count * sizeof(T);
         return reinterpret_cast<T*>(base + a);
     }

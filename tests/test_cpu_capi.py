@@ -134,3 +134,23 @@ def test_pgeof_import_name_is_a_drop_in():
     for name in pgeof.__all__:
         assert getattr(ref_name, name) is getattr(pgeof, name)
     assert int(ref_name.Eigentropy) == 13 and int(ref_name.Linearity) == 0
+
+
+def test_drop_in_surface_and_device_dispatch():
+    """`import pgeof` exposes the reference's names (src/pgeof/__init__.py:1-9)
+    and EFeatureID values at module level; host arrays (numpy, CPU tensors)
+    are not device arrays and go to the extension module."""
+    import torch
+
+    import pgeof
+    from point_geometric_features_amd import _device
+
+    for name in ("EFeatureID", "compute_features", "compute_features_multiscale", "compute_features_optimal",
+                 "knn_search", "radius_search", "compute_features_selected"):
+        assert hasattr(pgeof, name)
+    assert pgeof.Linearity == pgeof.EFeatureID.Linearity and int(pgeof.Eigentropy) == 13
+    assert not _device.is_device_array(np.zeros((4, 3), np.float32))
+    assert not _device.is_device_array(torch.zeros(4, 3))
+    assert not _device.is_device_array(None)
+    with pytest.raises(TypeError):
+        pgeof.compute_features(np.zeros((4, 3), np.float64), np.zeros(4, np.uint32), np.zeros(5, np.uint32))

@@ -60,3 +60,26 @@ def test_fused_knn_features_10m(scene10m, k):
     ref = O.compute_features(xyz, oi.ravel().copy(), ptr_s)
     val = O.eigvals_csr_np(xyz, oi.ravel().copy(), ptr_s)
     assert_features_close(f[sel].cpu().numpy(), ref, val, f"10M k={k}")
+
+
+def test_cross_cloud_knn_radius_1m():
+    """query != data at scale (include/nn_search.hpp:33, 88): 1 M data points,
+    1 M queries of which 20 % lie outside the data's bounding box (up to 30 m
+    beyond it); every row bit-exact against the oracle's KD-tree."""
+    data = scenes.lidar_scene(1_000_000, seed=111, extent=63.0)
+    rng = np.random.default_rng(112)
+    inside = scenes.lidar_scene(800_000, seed=113, extent=63.0)
+    outside = rng.uniform(-30.0, 93.0, size=(200_000, 3)).astype(np.float32)
+    outside[:, 2] = rng.uniform(-30.0, 45.0, 200_000)
+    q = np.concatenate([inside, outside])[rng.permutation(1_000_000)]
+    print("fullsize: cross-cloud 1M x 1M", flush=True)
+    import pgeof
+
+    gi, gd = pgeof.knn_search(data, q, 16)
+    oi, od = O.knn(data, q, 16, kdtree=True)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    ri, rd = pgeof.radius_search(data, q, 0.4, 32)
+    oi, od, _ = O.radius(data, q, 0.4, 32, kdtree=True)
+    np.testing.assert_array_equal(ri, oi)
+    np.testing.assert_array_equal(rd, od)

@@ -258,3 +258,19 @@ def test_tuning_knobs_change_speed_only(knobs):
     oi, od, _ = O.radius(xyz, xyz, 0.3, 24, kdtree=True)
     np.testing.assert_array_equal(ri, oi)
     np.testing.assert_array_equal(rd, od)
+
+
+def test_host_search_after_outputs_of_odd_length():
+    """The host entry points put the search workspace right after their
+    outputs; with m * k odd its base is 4 mod 8, and every workspace region
+    must still be aligned for the kernels' 64-bit atomics (regression: the
+    occupancy probe's sum of squares faulted there)."""
+    data = scenes.lidar_scene(80_000, seed=101, extent=25.0)
+    q = data[::7].copy()  # 11,429 queries
+    for k in (9, 7):
+        assert (len(q) * k) % 2 == 1
+        _eq_knn(data, q, k, kdtree=True)
+    ri, rd = pgeof.radius_search(data, q, 0.4, 9)
+    oi, od, _ = O.radius(data, q, 0.4, 9, kdtree=True)
+    np.testing.assert_array_equal(ri, oi)
+    np.testing.assert_array_equal(rd, od)
