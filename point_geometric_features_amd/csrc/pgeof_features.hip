@@ -42,13 +42,21 @@ __device__ __forceinline__ void features_of(const Moments& m, uint32_t k, float 
 // per-neighbour work light, so it takes whole rows of up to 32 at once; the
 // multiscale and optimal kernels evaluate features or eigentropies inside
 // the loop and keep their registers for that (measured best: 32 and 8).
-constexpr uint32_t kCsrChunkPlain = 32;
+#ifndef PGEOF_CSR_CHUNK
+#define PGEOF_CSR_CHUNK 32
+#endif
+#ifndef PGEOF_CSR_WAVES_PER_SIMD
+#define PGEOF_CSR_WAVES_PER_SIMD 1
+#endif
+constexpr uint32_t kCsrChunkPlain = PGEOF_CSR_CHUNK;
 constexpr uint32_t kCsrChunkScales = 8;
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_csr(CsrArgs a)
+__global__ __launch_bounds__(256, PGEOF_CSR_WAVES_PER_SIMD) void k_csr(CsrArgs a)
 {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    // consecutive blocks of rows (neighbours in space when an order is
+    // given) run on the same XCD and share its L2
+    const uint32_t t = xcd_logical_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
     if (t >= a.n_points) return;
     // rows are visited in the spatial order of their points when an
     // order is given, so that a wave's gathers share cache lines
@@ -126,13 +134,31 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
     // with one chunk of 32).
     constexpr uint32_t kChunk = MODE == kCsrPlain ? kCsrChunkPlain : kCsrChunkScales;
     uint32_t nid[kChunk];
+    // a row that starts on a 16-byte boundary (every row of a kNN CSR with k
+    // a multiple of 4) is read four indices per load
+    const bool row16 = (reinterpret_cast<uintptr_t>(row) & 15u) == 0;
     auto load_idx = [&](uint32_t j0) {
+        static_assert(kChunk % 4 == 0, "chunks are read in groups of four");
 #pragma unroll
-        for (uint32_t u = 0; u < kChunk; ++u)
+        for (uint32_t u = 0; u < kChunk; u += 4)
         {
-            uint32_t idx = (j0 + u < walk) ? row[j0 + u] : i0;
-            bad |= idx >= a.n_xyz;
-            nid[u] = idx < a.n_xyz ? idx : 0u;
+            uint32_t v[4];
+            if (row16 && j0 + u + 3 < walk)
+            {
+                const uint4 q = *reinterpret_cast<const uint4*>(row + j0 + u);
+                v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+            }
+            else
+            {
+#pragma unroll
+                for (uint32_t w = 0; w < 4; ++w) v[w] = (j0 + u + w < walk) ? row[j0 + u + w] : i0;
+            }
+#pragma unroll
+            for (uint32_t w = 0; w < 4; ++w)
+            {
+                bad |= v[w] >= a.n_xyz;
+                nid[u + w] = v[w] < a.n_xyz ? v[w] : 0u;
+            }
         }
     };
     load_idx(0);
@@ -159,8 +185,7 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
                 while (s < a.n_scales && a.scales[s] == cnt)
                 {
                     features_of(m, cnt, f);
-                    float* o = out + (size_t)s * 11;
-                    for (int c = 0; c < 11; ++c) o[c] = f[c];
+                    store_row11(out + (size_t)s * 11, f);
                     ++s;
                 }
             }
@@ -191,7 +216,7 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
     if (MODE == kCsrPlain)
     {
         features_of(m, walk, f);
-        for (int c = 0; c < 11; ++c) out[c] = f[c];
+        store_row11(out, f);
     }
     else if (MODE == kCsrMulti)
     {
@@ -199,26 +224,32 @@ __global__ __launch_bounds__(256) void k_csr(CsrArgs a)
     }
     else
     {
+        // rows of 12 floats are 16-byte aligned: three 16-byte stores
         features_of(best, k_opt, f);
-        for (int c = 0; c < 11; ++c) out[c] = f[c];
-        out[11] = (float)k_opt;
+        float4* o4 = reinterpret_cast<float4*>(out);
+        o4[0] = make_float4(f[0], f[1], f[2], f[3]);
+        o4[1] = make_float4(f[4], f[5], f[6], f[7]);
+        o4[2] = make_float4(f[8], f[9], f[10], (float)k_opt);
     }
 }
 
 // ---- spatial order of CSR rows ----
-// Key of a row: the Morton code (kMortonBits per axis), in the bounding box
-// of xyz (read from device memory, no host round trip), of the row's own
-// point (row i of a pgeof CSR is point i, whose first neighbour in a search
-// of the cloud against itself is the point itself), read as a stream; a row
-// past the end of xyz takes its first neighbour instead, and an empty or
-// invalid one 0 (the feature kernel reports it).
-constexpr int kMortonBits = 7;  // 2^21 buckets
+// Key of a row: a Morton-like code of kOrderBits bits, in the bounding box of
+// xyz (read from device memory, no host round trip), of the row's own point
+// (row i of a pgeof CSR is point i, whose first neighbour in a search of the
+// cloud against itself is the point itself), read as a stream; a row past the
+// end of xyz takes its first neighbour instead, and an empty or invalid one 0
+// (the feature kernel reports it).  The bits are handed out one at a time to
+// the axis whose cells are currently the longest, so the finest cells are
+// about cubic whatever the box's shape: a flat scene (200 m x 200 m x 20 m)
+// gets most bits in x and y.  Equal bits per axis would leave the ground of
+// such a scene in one z slice, with hundreds of rows per bucket and the
+// histogram's atomics serialised on them.
+constexpr int kOrderBits = 21;  // 2^21 buckets
 
-__device__ __forceinline__ uint32_t spread3(uint32_t v)
+__device__ __forceinline__ int longest_axis(const float cell[3])
 {
-    uint32_t r = 0;
-    for (int b = 0; b < kMortonBits; ++b) r |= ((v >> b) & 1u) << (3 * b);
-    return r;
+    return cell[0] >= cell[1] ? (cell[0] >= cell[2] ? 0 : 2) : (cell[1] >= cell[2] ? 1 : 2);
 }
 
 __global__ __launch_bounds__(256) void k_row_keys(const float* __restrict__ xyz, uint32_t n_xyz,
@@ -236,19 +267,38 @@ __global__ __launch_bounds__(256) void k_row_keys(const float* __restrict__ xyz,
         j = (b < e && (uint64_t)e <= n_nn) ? nn[b] : n_xyz;
     }
     uint32_t k = 0;
+    if (j < n_xyz)
     {
-        if (j < n_xyz)
+        float lo[3], ext[3], cell[3];
+        int bits[3] = {0, 0, 0}, used[3] = {0, 0, 0};
+        for (int ax = 0; ax < 3; ++ax)
         {
-            const float side = (float)(1u << kMortonBits);
-            uint32_t c[3];
-            for (int ax = 0; ax < 3; ++ax)
-            {
-                const float lo = bbox[ax], ext = bbox[3 + ax] - bbox[ax];
-                float f = ext > 0.0f ? (xyz[(size_t)j * 3 + ax] - lo) / ext * side : 0.0f;
-                f = fminf(fmaxf(f, 0.0f), side - 1.0f);  // also maps NaN to 0
-                c[ax] = (uint32_t)f;
-            }
-            k = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+            lo[ax] = bbox[ax];
+            ext[ax] = fmaxf(bbox[3 + ax] - bbox[ax], 0.0f);
+            cell[ax] = ext[ax];
+        }
+        for (int b = 0; b < kOrderBits; ++b)
+        {
+            const int ax = longest_axis(cell);
+            ++bits[ax];
+            cell[ax] *= 0.5f;
+        }
+        uint32_t c[3];
+        for (int ax = 0; ax < 3; ++ax)
+        {
+            const float side = (float)(1u << bits[ax]);
+            float f = ext[ax] > 0.0f ? (xyz[(size_t)j * 3 + ax] - lo[ax]) / ext[ax] * side : 0.0f;
+            f = fminf(fmaxf(f, 0.0f), side - 1.0f);  // also maps NaN to 0
+            c[ax] = (uint32_t)f;
+            cell[ax] = ext[ax];
+        }
+        // the bits in the order they were handed out, coarsest first
+        for (int b = 0; b < kOrderBits; ++b)
+        {
+            const int ax = longest_axis(cell);
+            k = (k << 1) | ((c[ax] >> (bits[ax] - 1 - used[ax])) & 1u);
+            ++used[ax];
+            cell[ax] *= 0.5f;
         }
     }
     key[i] = k;
@@ -267,7 +317,7 @@ __global__ __launch_bounds__(256) void k_key_scatter(const uint32_t* __restrict_
 
 size_t csr_order_bytes(uint32_t n_points)
 {
-    const uint64_t buckets = 1ull << (3 * kMortonBits);
+    const uint64_t buckets = 1ull << kOrderBits;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     return al(1024 * 6 * 4) + al(16 * 4) + 3 * al((size_t)n_points * 4) + al((buckets + 1) * 4) +
            al(scan_partials_needed(buckets + 1) * 4);
@@ -276,7 +326,7 @@ size_t csr_order_bytes(uint32_t n_points)
 hipError_t launch_csr_order(const CsrArgs& a, void* ws, uint32_t* order_out_ptr_holder[1], hipStream_t s)
 {
     const uint32_t n = a.n_points;
-    const uint64_t buckets = 1ull << (3 * kMortonBits);
+    const uint64_t buckets = 1ull << kOrderBits;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     char* p = (char*)ws;
     float* bb_part = (float*)p;

@@ -1517,7 +1517,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                         const Pca32 P = to_pca32(E);
                         float f[11];
                         features11(P, f);
-                        for (int cc = 0; cc < 11; ++cc) of[(size_t)si * 11 + cc] = f[cc];
+                        store_row11(of + (size_t)si * 11, f);
                         ++si;
                     }
                 }
@@ -1611,8 +1611,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
             }
             for (int c = 0; c < 11; ++c) f[c] = 0.0f;
         }
-        float* of = a.out_feat + (size_t)qorig * 11;
-        for (int c = 0; c < 11; ++c) of[c] = f[c];
+        store_row11(a.out_feat + (size_t)qorig * 11, f);
     }
 }
 
