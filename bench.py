@@ -192,22 +192,44 @@ def _copy_gbs(torch, dev) -> float:
 def _serving(args, capi, scenes):
     """The serving configuration: a stream of `--requests` clouds (sizes
     log-uniform in 5k..2M, windows of a LiDAR-like scene), knn k=20 +
-    features each, through the serving loop (4 slots, 64 GB pool).  Three
-    runs over the same stream: one request at a time (each submitted when the
-    previous one has completed: the plain single-request path), closed loop
-    (all submitted at once: the capacity) and open loop (Poisson arrivals at
-    0.8 of that capacity).  Latency runs from submission (arrival) to the
-    result in the caller's array."""
+    features each, with the clouds and the results in pinned host memory.
+
+    * plain path, one request at a time: per request the cloud is copied in,
+      knn_search on the device (pgeof_knn_search_device), the CSR nn_ptr =
+      arange * k, compute_features on the device
+      (pgeof_compute_features_device), the features copied out, then the next
+      request (the reference's two calls, GPU-resident, no serving loop);
+    * the serving loop (pgeof_server_*: 4 slots, streams, HBM pool, fused
+      kNN + features): one request at a time, closed loop (all submitted at
+      once: the capacity) and open loop (Poisson arrivals at 0.8 of the
+      capacity; latency from the scheduled arrival to the result in the
+      caller's array); the open loop again at half the capacity."""
+    import torch
+
     k = 20
     clouds = [c for c in scenes.request_stream(args.requests, seed=4) if len(c) >= k]
-    n_pts = sum(len(c) for c in clouds)
-    _progress(f"serving: {len(clouds)} requests generated ({n_pts} points)")
+    sizes = np.array([len(c) for c in clouds], np.int64)
+    n_pts = int(sizes.sum())
+    starts = np.r_[0, np.cumsum(sizes)]
+    # every cloud in one pinned buffer, results in a pool of pinned buffers
+    pin_in = torch.empty((n_pts, 3), dtype=torch.float32, pin_memory=True)
+    pin_np = pin_in.numpy()
+    for i, c in enumerate(clouds):
+        pin_np[starts[i]:starts[i + 1]] = c
+    del clouds
+    req = [pin_np[starts[i]:starts[i + 1]] for i in range(len(sizes))]
+    n_max = int(sizes.max())
+    n_out_buf = 48
+    out_pool = [torch.empty((n_max, 11), dtype=torch.float32, pin_memory=True).numpy() for _ in range(n_out_buf)]
+    _progress(f"serving: {len(req)} requests generated ({n_pts} points, pinned)")
     budget = 64 << 30
 
-    def run(subset, n_slots, arrivals=None, sequential=False):
-        lat = {}
+    def run(n_slots, arrivals=None, sequential=False, count=None):
+        ids = range(len(req) if count is None else count)
+        done_at, sub_at = {}, {}
+        free = list(range(n_out_buf))
+        held = {}
         with capi.Server(n_slots, budget, k) as srv:
-            outs = {}
             t0 = time.perf_counter()
             pending = 0
 
@@ -216,11 +238,11 @@ def _serving(args, capi, scenes):
                 for rid, st, ms in srv.wait(max_items, timeout_ms=timeout_ms):
                     if st != 0:
                         raise RuntimeError(f"serving request {rid} failed with status {st}")
-                    lat[rid] = ms
+                    done_at[rid] = sub_at[rid] + ms * 1e-3
                     pending -= 1
-                    outs.pop(rid, None)
+                    free.append(held.pop(rid))
 
-            for i, c in enumerate(subset):
+            for i in ids:
                 if sequential:
                     while pending:
                         drain(1, 60_000)
@@ -230,45 +252,87 @@ def _serving(args, capi, scenes):
                         if dt <= 0:
                             break
                         drain(64, dt * 1e3)  # completions while waiting for the next arrival
-                outs[i] = np.empty((len(c), 11), np.float32)
-                srv.submit(i, c, outs[i])
+                while not free:
+                    drain(1, 60_000)
+                b = free.pop()
+                held[i] = b
+                sub_at[i] = time.perf_counter() - t0
+                srv.submit(i, req[i], out_pool[b][: sizes[i]])
                 pending += 1
             while pending:
                 drain(64, 60_000)
             wall = time.perf_counter() - t0
-        return wall, np.array([lat[i] for i in range(len(subset))])
+        t_from = arrivals if arrivals is not None else sub_at
+        return wall, np.array([(done_at[i] - t_from[i]) * 1e3 for i in ids])
+
+    def run_plain(count=None):
+        ids = range(len(req) if count is None else count)
+        dev = torch.device("cuda:0")
+        x = torch.empty((n_max, 3), dtype=torch.float32, device=dev)
+        idx = torch.empty((n_max, k), dtype=torch.int32, device=dev)
+        ptr = torch.arange(n_max + 1, dtype=torch.int32, device=dev) * k
+        f = torch.empty((n_max, 11), dtype=torch.float32, device=dev)
+        ws = torch.empty(capi.knn_workspace_size(n_max, n_max, k), dtype=torch.uint8, device=dev)
+        lat = []
+        t0 = time.perf_counter()
+        for i in ids:
+            ts = time.perf_counter()
+            n = int(sizes[i])
+            xi = x[:n]
+            xi.copy_(pin_in[starts[i]:starts[i + 1]], non_blocking=True)
+            capi.knn_search_device(xi, xi, k, idx[:n], workspace=ws)
+            capi.compute_features_device(xi, idx[:n].view(-1), ptr[: n + 1], f[:n])
+            out = torch.from_numpy(out_pool[0][:n])
+            out.copy_(f[:n], non_blocking=True)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - ts)
+        return time.perf_counter() - t0, np.array(lat) * 1e3
 
     def pct(a, q):
         return round(float(np.percentile(a, q)), 2)
 
-    run(clouds[:64], 4)  # warm-up (code objects, pool allocation)
-    wall_seq, lat_seq = run(clouds, 1, sequential=True)
-    _progress(f"serving: one request at a time {wall_seq:.2f} s")
-    wall_cap, lat_cap = run(clouds, 4)
+    run(4, count=64)  # warm-up (code objects, pool allocation)
+    run_plain(count=16)
+    wall_plain, lat_plain = run_plain()
+    _progress(f"serving: plain path one request at a time {wall_plain:.2f} s")
+    wall_seq, lat_seq = run(1, sequential=True)
+    _progress(f"serving: serving loop one request at a time {wall_seq:.2f} s")
+    wall_cap, lat_cap = run(4)
     _progress(f"serving: closed loop {wall_cap:.2f} s")
     cap_rate = n_pts / wall_cap
-    rate_req = 0.8 * cap_rate / (n_pts / len(clouds))
-    gaps = np.random.default_rng(5).exponential(1.0 / rate_req, size=len(clouds))
+    rate_req = 0.8 * cap_rate / (n_pts / len(req))
+    gaps = np.random.default_rng(5).exponential(1.0 / rate_req, size=len(req))
     arrivals = np.cumsum(gaps) - gaps[0]
-    wall_open, lat_open = run(clouds, 4, arrivals)
+    wall_open, lat_open = run(4, arrivals)
     _progress(f"serving: open loop {wall_open:.2f} s")
-    seq_rate, open_rate = n_pts / wall_seq, n_pts / wall_open
+    # the same stream at half the capacity (shorter queues, lower p99)
+    wall_half, lat_half = run(4, arrivals / 0.625)
+    _progress(f"serving: open loop at half capacity {wall_half:.2f} s")
+    plain_rate, seq_rate, open_rate = n_pts / wall_plain, n_pts / wall_seq, n_pts / wall_open
     # "p99-latency-adjusted throughput": points/s divided by the p99 latency
     # (s), i.e. throughput that counts each result only as fast as the
-    # slowest 1 % arrive; the ratio compares the open-loop serving stream
-    # with the one-request-at-a-time path over the same requests
-    adj_seq = seq_rate / (np.percentile(lat_seq, 99) * 1e-3)
-    adj_open = open_rate / (np.percentile(lat_open, 99) * 1e-3)
+    # slowest 1 % arrive
+    adj = lambda rate, lat: rate / (np.percentile(lat, 99) * 1e-3)  # noqa: E731
     return {
-        "requests": len(clouds), "points": int(n_pts), "slots": 4, "hbm_pool_gb": budget >> 30,
+        "requests": len(req), "points": n_pts, "slots": 4, "hbm_pool_gb": budget >> 30,
+        "host_buffers": "pinned",
         "capacity_mpts_s": round(cap_rate / 1e6, 2),
+        "plain_path_one_at_a_time_mpts_s": round(plain_rate / 1e6, 2),
+        "plain_path_latency_ms": {"p50": pct(lat_plain, 50), "p99": pct(lat_plain, 99)},
         "one_at_a_time_mpts_s": round(seq_rate / 1e6, 2),
         "one_at_a_time_latency_ms": {"p50": pct(lat_seq, 50), "p99": pct(lat_seq, 99)},
         "open_loop_rate_req_s": round(rate_req, 2),
         "open_loop_mpts_s": round(open_rate / 1e6, 2),
         "open_loop_latency_ms": {"p50": pct(lat_open, 50), "p99": pct(lat_open, 99)},
+        "open_loop_half_load_mpts_s": round(n_pts / wall_half / 1e6, 2),
+        "open_loop_half_load_latency_ms": {"p50": pct(lat_half, 50), "p99": pct(lat_half, 99)},
+        "throughput_ratio_open_vs_plain_path": round(open_rate / plain_rate, 3),
+        "throughput_ratio_capacity_vs_plain_path": round(cap_rate / plain_rate, 3),
         "throughput_ratio_open_vs_one_at_a_time": round(open_rate / seq_rate, 3),
-        "p99_adjusted_throughput_ratio": round(adj_open / adj_seq, 3),
+        "p99_adjusted_throughput_ratio_vs_plain_path": round(adj(open_rate, lat_open) / adj(plain_rate, lat_plain), 3),
+        "p99_adjusted_throughput_ratio_half_load_vs_plain_path": round(
+            adj(n_pts / wall_half, lat_half) / adj(plain_rate, lat_plain), 3),
+        "p99_adjusted_throughput_ratio": round(adj(open_rate, lat_open) / adj(seq_rate, lat_seq), 3),
     }
 
 
