@@ -457,7 +457,7 @@ def main():
     args = _parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
-                                   "(the multi-GPU path is a design only, see DESIGN.md)"}))
+                                   "(the multi-GPU path is a design only)"}))
         sys.exit(2)
 
     import point_geometric_features_amd as pgeof  # noqa: F401  (selects the shared HIP runtime)

@@ -70,7 +70,7 @@ def _features_closed_form(lam, v2):
 
 
 def test_features_closed_forms_collinear_planar_zero():
-    # collinear along x, variance 1.25 (SURVEY closed form, Eigen ascending order)
+    # collinear along x, variance 1.25 (closed form with Eigen's ascending order: lambda = (0, 0, 1.25))
     line = np.zeros((4, 3), np.float32)
     line[:, 0] = [-1.5, -0.5, 0.5, 1.5]
     # planar z = 2 with variances 1 (x) and 4 (y)
