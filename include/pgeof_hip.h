@@ -249,7 +249,10 @@ void pgeof_server_destroy(pgeof_server* server);
  * plain and optimal-k kernels visit the rows of inputs of 65536 rows or more
  * in the spatial order of their points, 0: in index order), "heap_tail"
  * (1: search heaps of k > 21 keep nodes 21..k-1 in global memory and only
- * nodes 5..20 in LDS, 0: the whole heap in LDS while it fits), "staging"
+ * nodes 5..20 in LDS; searches of over 2 M queries keep those tails per
+ * hardware wave slot, 128 MiB per 4 tail nodes, smaller ones per query;
+ * 2: per hardware wave slot at any size; 0: the whole heap in LDS while it
+ * fits), "staging"
  * (wave-shared LDS staging of cell rows: 0 never, 1 radius search and,
  * for k >= 32, the budgeted kNN pass, 2 radius search and the budgeted kNN
  * pass at any k), "fuse_multiscale" (pgeof_radius_multiscale_device: 1 the

@@ -64,7 +64,7 @@ struct Tuning
     double budget_levels = 1.0;         // pyramid levels that get a budgeted pass (the rest: unbudgeted only)
     double grid_level_ratio = 2.0;      // cell edge ratio between consecutive pyramid levels (2..8)
     double csr_order = 1.0;             // CSR features: visit large inputs' rows in spatial order
-    double heap_tail = 1.0;             // search heaps: nodes past 20 in global memory (k > 21)
+    double heap_tail = 1.0;             // search heaps: nodes past 20 in global memory (k > 21); 2: per wave slot always
     double staging = 1.0;               // LDS staging of cell rows: 0 never, 1 radius + budgeted kNN at k >= 32, 2 always
     double fuse_multiscale = 0.0;       // radius + multiscale: features in the search kernel (1) or a CSR kernel after it (0)
 };
@@ -185,15 +185,17 @@ std::map<std::string, std::pair<int64_t, double>> g_prof_acc;
 struct DevState
 {
     int lds_max = 65536;
+    int xccs = 0;  // 0: unknown
     bool init = false;
 };
 std::mutex g_dev_mu;
 std::map<int, DevState> g_dev;
 
-int lds_limit()
+// attributes of the current device (defaults when there is none)
+DevState dev_state()
 {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipGetDevice(&dev) != hipSuccess) return DevState{};
     std::lock_guard<std::mutex> l(g_dev_mu);
     DevState& d = g_dev[dev];
     if (!d.init)
@@ -201,9 +203,21 @@ int lds_limit()
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
             d.lds_max = v;
+        v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeNumberOfXccs, dev) == hipSuccess && v > 0) d.xccs = v;
         d.init = true;
     }
-    return d.lds_max;
+    return d;
+}
+
+int lds_limit() { return dev_state().lds_max; }
+
+// whether hw_wave_slot() names every resident wave of the current device
+// uniquely (its XCC id fits the bits it keeps)
+bool hw_slots_ok()
+{
+    const int x = dev_state().xccs;
+    return x > 0 && x <= (int)kHwMaxXcc;
 }
 
 // Scratch of device calls made without a workspace: device buffers kept by
@@ -501,13 +515,32 @@ HeapPlan heap_plan(uint32_t k)
     return {256, 0, true, 0u};
 }
 
+// Where the heap tails of a search over m queries live: 2 one per hardware
+// wave slot, 1 one per lane of the launch, 0 none.  Per wave slot, only the
+// slots of the resident waves are touched (a few MB, which stay in cache,
+// instead of a tail per query, rewritten through to HBM); its storage,
+// kHwWaveSlots * 64 lanes, is taken once the launch has at least half as
+// many lanes, or always with heap_tail 2.
+int tail_layout(const HeapPlan& hp, uint64_t m)
+{
+    if (!hp.tail_pairs) return 0;
+    if (!hw_slots_ok()) return 1;
+    const bool force = tuning().heap_tail > 1.5;
+    return force || (m + 256) * 2 >= (uint64_t)kHwWaveSlots * 64 ? 2 : 1;
+}
+
 // global-memory heap storage of a search over m queries: the whole heap
-// (fallback) or the tail of every lane of the launch (grid rounded up to 256)
+// (fallback), or the tail of every hardware wave slot, or of every lane of
+// the launch (grid rounded up to 256)
 size_t heap_global_bytes(const HeapPlan& hp, uint64_t m, uint32_t k)
 {
     if (hp.global) return (size_t)m * k * 8;
-    if (hp.tail_pairs) return (size_t)(m + 256) * hp.tail_pairs * 16;
-    return 0;
+    switch (tail_layout(hp, m))
+    {
+        case 2: return (size_t)kHwWaveSlots * 64 * hp.tail_pairs * 16;
+        case 1: return (size_t)(m + 256) * hp.tail_pairs * 16;
+        default: return 0;
+    }
 }
 
 size_t search_ws_bytes(uint64_t n, uint64_t m, uint32_t k, bool csr)
@@ -686,7 +719,8 @@ int run_search(int mode, const float* data, uint32_t n, const float* query, uint
     a.out_count = csr ? w.counts : out_count;
     const HeapPlan hp = heap_plan(k);
     a.gheap_keys = w.gheap_keys;
-    a.heap_tail = hp.tail_pairs ? 1 : 0;
+    a.heap_tail = tail_layout(hp, m);
+    a.tail_pairs = hp.tail_pairs;
     if (mode == kModeKnn || mode == kModeKnnFeat)
     {
         // Budgeted passes, finest level first: a query whose k-th neighbour

@@ -46,6 +46,12 @@ struct Level
 
 constexpr int kMaxLevels = 4;
 
+// Hardware wave slots a kernel can name with hw_wave_slot(): 3 bits of XCC,
+// 2 of shader engine, 1 of shader array, 4 of CU, 2 of SIMD and 4 of wave
+// slot.  Waves resident at the same time never share one.
+constexpr uint32_t kHwWaveSlots = 1u << 16;
+constexpr uint32_t kHwMaxXcc = 8;
+
 // The grid pyramid of one search, planned and written by kernels (bounding
 // box, sizing probe, levels), so that no value travels to the host between
 // the launches of a search.  Level l has cell edge h_0 * ratio^l; a level
@@ -96,7 +102,12 @@ struct SearchArgs
     float* out_feat;      // (n_query, 11) or nullptr (fused features)
     uint32_t* out_count;  // per original query: number of hits (radius) or nullptr
     uint64_t* gheap_keys;  // global-memory heap fallback: k * n_query keys; or, with heap_tail, heap nodes past 20
-    int heap_tail;         // heap nodes past 20 in gheap_keys: pair p of lane t at (p * lanes + t) * 2
+    // heap nodes past 20 in gheap_keys, as 16-byte pairs: 0 none; 1 one slot
+    // per lane of the launch, pair p of lane t at (p * lanes + t) * 2; 2 one
+    // slot per hardware wave slot (kHwWaveSlots), pair p of lane l of wave
+    // slot w at ((w * tail_pairs + p) * 64 + l) * 2
+    int heap_tail;
+    uint32_t tail_pairs;   // heap_tail 2: pairs per lane
     uint32_t* defer_flag;  // budgeted pass: 1 where the query is left to the next pass
     const float* query_bound;  // per query: upper bound of its k-th squared distance, or +inf (nullable)
     float* defer_bound;        // budgeted pass: that bound for the deferred queries

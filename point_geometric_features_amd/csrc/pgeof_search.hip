@@ -177,7 +177,9 @@ __global__ __launch_bounds__(256) void k_hist(const float* __restrict__ xyz, uin
         if (i < n)
         {
             cid[u] = cell_id(L, (double)xyz[i * 3 + 0], (double)xyz[i * 3 + 1], (double)xyz[i * 3 + 2]);
-            lr[u] = atomicAdd(small ? &bins[cid[u]] : &count[cid[u]], 1u);
+            // two atomics, not one through a selected pointer: that pointer
+            // would be generic (a flat atomic) instead of an LDS or global one
+            lr[u] = small ? atomicAdd(&bins[cid[u]], 1u) : atomicAdd(&count[cid[u]], 1u);
         }
     }
     if (small)
@@ -591,6 +593,41 @@ __global__ __launch_bounds__(256) void k_compact_rows(const uint32_t* __restrict
     }
 }
 
+// ------------------------------------------------- global-memory access ----
+// Pointers that reach a kernel through a table in device memory (the levels
+// of the pyramid, moved to scalar registers) lose their address space, and
+// the compiler then reads them with flat loads.  A flat load also counts
+// against the LDS wait counter, so every LDS access of the heaps would wait
+// for the candidate loads in flight; these helpers read and write through
+// global-address-space pointers instead (global_load / global_store).
+#define PGEOF_GLOBAL __attribute__((address_space(1)))
+typedef float pgeof_f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long pgeof_u64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float4 gload(const float4* p, size_t i)
+{
+    const pgeof_f32x4 v = ((const PGEOF_GLOBAL pgeof_f32x4*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t gload(const uint32_t* p, size_t i)
+{
+    return ((const PGEOF_GLOBAL uint32_t*)p)[i];
+}
+__device__ __forceinline__ uint64_t gload(const uint64_t* p, size_t i)
+{
+    return ((const PGEOF_GLOBAL unsigned long long*)p)[i];
+}
+// two consecutive 8-byte entries at p (16-byte aligned)
+__device__ __forceinline__ ulonglong2 gload2(const uint64_t* p)
+{
+    const pgeof_u64x2 v = *(const PGEOF_GLOBAL pgeof_u64x2*)p;
+    return make_ulonglong2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore(uint64_t* p, size_t i, uint64_t v)
+{
+    ((PGEOF_GLOBAL unsigned long long*)p)[i] = v;
+}
+
 // --------------------------------------------------------------- heap ----
 // Keys are 64-bit: squared-distance bits << 32 | original index, so that the
 // order of keys is the order of (distance, index) and ties go to the smaller
@@ -681,10 +718,11 @@ struct Heap
 // conflict-free 16-byte loads.  A sift from the root of a 20-entry heap reads
 // four registers and one LDS group; the binary-heap equivalent walks four
 // levels of LDS.  With TAIL (large k) only nodes 5..20 (groups 0..3, 128
-// bytes per lane) are in LDS and nodes 21..k-1 in global memory, laid out
-// like the LDS pairs with the launch's lanes in place of the block's: LDS
-// then stays small enough for full occupancy whatever k is, and only sifts
-// that reach the fourth level touch global memory.
+// bytes per lane) are in LDS and nodes 21..k-1 in global memory as
+// lane-interleaved 16-byte pairs, in the slot of the lane's hardware wave
+// slot or of its lane of the launch: LDS then stays small enough for full
+// occupancy whatever k is, and only sifts that reach the fourth level touch
+// global memory.
 template <bool TAIL>
 struct Heap4
 {
@@ -693,7 +731,7 @@ struct Heap4
     uint32_t k;
     uint32_t count;    // keys inserted, up to k
     uint64_t* gkeys;   // this lane's first global pair; pair p at gkeys + p * gpstride
-    size_t gpstride;   // entries between consecutive global pairs of one lane (2 * lanes of the launch)
+    size_t gpstride;   // entries between consecutive global pairs of one lane
     uint64_t rkey;     // node 0
     uint64_t n1, n2, n3, n4;  // nodes 1..4 (separate scalars, never an indexed array)
 
@@ -714,14 +752,14 @@ struct Heap4
     }
     __device__ __forceinline__ uint64_t load(uint32_t i) const  // i >= 5
     {
-        return in_lds(i) ? keys[slot(i)] : gkeys[gslot(i)];
+        return in_lds(i) ? keys[slot(i)] : gload(gkeys, gslot(i));
     }
     __device__ __forceinline__ void store(uint32_t i, uint64_t kk)  // i >= 5
     {
         if (in_lds(i))
             keys[slot(i)] = kk;
         else
-            gkeys[gslot(i)] = kk;
+            gstore(gkeys, gslot(i), kk);
     }
     // The register nodes are combined with bit masks, never indexed or
     // stored in a branch: a select chain over consecutive members would be
@@ -764,8 +802,8 @@ struct Heap4
         else
         {
             const uint32_t gg = g - (kLdsTop - 4) / 4;
-            g01 = *reinterpret_cast<const ulonglong2*>(gkeys + (size_t)(2 * gg) * gpstride);
-            g23 = *reinterpret_cast<const ulonglong2*>(gkeys + (size_t)(2 * gg + 1) * gpstride);
+            g01 = gload2(gkeys + (size_t)(2 * gg) * gpstride);
+            g23 = gload2(gkeys + (size_t)(2 * gg + 1) * gpstride);
         }
         uint64_t mk = g01.x;
         uint32_t mj = 0;
@@ -861,6 +899,21 @@ struct Heap4
         }
     }
 };
+
+// The hardware wave slot this wave occupies, < kHwWaveSlots: XCC (XCC_ID
+// register, bits 3:0, at most kHwMaxXcc of them), shader engine (HW_ID bits
+// 14:13), shader array (12), CU (11:8), SIMD (5:4) and wave buffer slot
+// (3:0).  Two waves resident at the same time have different slots, so a
+// slot's memory is private to the wave that holds it until the wave ends.
+__device__ __forceinline__ uint32_t hw_wave_slot()
+{
+    constexpr int kHwIdReg = 4, kXccIdReg = 20;
+    const uint32_t hw = __builtin_amdgcn_s_getreg(((15 - 1) << 11) | (0 << 6) | kHwIdReg);
+    const uint32_t xcc = __builtin_amdgcn_s_getreg(((4 - 1) << 11) | (0 << 6) | kXccIdReg);
+    const uint32_t wave = hw & 15u, simd = (hw >> 4) & 3u, cu = (hw >> 8) & 15u, sh = (hw >> 12) & 1u,
+                   se = (hw >> 13) & 3u;
+    return (((((xcc & (kHwMaxXcc - 1)) * 4u + se) * 2u + sh) * 16u + cu) * 4u + simd) * 16u + wave;
+}
 
 // distance from v to the slab [lo + c h, lo + (c+1) h], less a safety margin
 __device__ __forceinline__ double axis_gap(double v, int c, double lo, double h, double margin)
@@ -1007,10 +1060,21 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
         heap.pstride = 2 * BLOCK;
         if constexpr (HEAP == kHeapTail)
         {
-            // nodes past 20 in global memory (lane-interleaved pairs, one
-            // slot per hardware lane of the launch)
-            heap.gkeys = a.gheap_keys + ((size_t)blockIdx.x * BLOCK + tid) * 2;
-            heap.gpstride = (size_t)2 * gridDim.x * (size_t)BLOCK;
+            // nodes past 20 in global memory as lane-interleaved pairs: one
+            // slot per hardware wave slot (a wave's pairs are contiguous and
+            // the few MB of resident slots stay in cache) or, for small
+            // searches, one slot per lane of the launch
+            if (a.heap_tail == 2)
+            {
+                const uint32_t w = hw_wave_slot();
+                heap.gkeys = a.gheap_keys + (((size_t)w * a.tail_pairs) * 64 + (size_t)lane) * 2;
+                heap.gpstride = 128;
+            }
+            else
+            {
+                heap.gkeys = a.gheap_keys + ((size_t)blockIdx.x * BLOCK + tid) * 2;
+                heap.gpstride = (size_t)2 * gridDim.x * (size_t)BLOCK;
+            }
         }
     }
     heap.k = a.k;
@@ -1063,7 +1127,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                 for (int y = max(ly - 1, 0); y <= min(ly + 1, L.g[1] - 1); ++y)
                 {
                     const uint32_t base = ((uint32_t)z * (uint32_t)L.g[1] + (uint32_t)y) * (uint32_t)L.g[0];
-                    count += L.cs[base + (uint32_t)x1 + 1] - L.cs[base + (uint32_t)x0];
+                    count += gload(L.cs, base + (uint32_t)x1 + 1) - gload(L.cs, base + (uint32_t)x0);
                 }
             if (count >= a.k) break;
         }
@@ -1107,7 +1171,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
             {
                 float4 p[kScanUnroll];
 #pragma unroll
-                for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = pts[min(j + u, hi - 1)];
+                for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = gload(pts, min(j + u, hi - 1));
 #pragma unroll
                 for (uint32_t u = 0; u < kScanUnroll; ++u)
                     if (j + u < hi) consider(p[u], j + u);
@@ -1210,13 +1274,13 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                     uint32_t s0 = 0xFFFFFFFFu, e0 = 0, s1 = 0xFFFFFFFFu, e1 = 0;
                     if (a0 <= b0)
                     {
-                        s0 = cs[base + (uint32_t)a0];
-                        e0 = cs[base + (uint32_t)b0 + 1];
+                        s0 = gload(cs, base + (uint32_t)a0);
+                        e0 = gload(cs, base + (uint32_t)b0 + 1);
                     }
                     if (a1 <= b1)
                     {
-                        s1 = cs[base + (uint32_t)a1];
-                        e1 = cs[base + (uint32_t)b1 + 1];
+                        s1 = gload(cs, base + (uint32_t)a1);
+                        e1 = gload(cs, base + (uint32_t)b1 + 1);
                     }
                     const uint32_t S = wave_min_u(min(s0 < e0 ? s0 : 0xFFFFFFFFu, s1 < e1 ? s1 : 0xFFFFFFFFu));
                     const uint32_t E = wave_max_u(max(s0 < e0 ? e0 : 0u, s1 < e1 ? e1 : 0u));
@@ -1227,7 +1291,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                         for (int u = 0; u < kStage / 64; ++u)
                         {
                             const uint32_t i = (uint32_t)(u * 64 + lane);
-                            if (i < cnt) stage[i] = wpts[c + i];
+                            if (i < cnt) stage[i] = gload(wpts, c + i);
                         }
                         wave_sync();
                         scan_staged(s0, e0, s1, e1, c, cnt);
@@ -1262,13 +1326,13 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                     uint32_t s0 = 0, e0 = 0, s1 = 0, e1 = 0;
                     if (a0 <= b0)
                     {
-                        s0 = cs[base + (uint32_t)a0];
-                        e0 = cs[base + (uint32_t)b0 + 1];
+                        s0 = gload(cs, base + (uint32_t)a0);
+                        e0 = gload(cs, base + (uint32_t)b0 + 1);
                     }
                     if (a1 <= b1)
                     {
-                        s1 = cs[base + (uint32_t)a1];
-                        e1 = cs[base + (uint32_t)b1 + 1];
+                        s1 = gload(cs, base + (uint32_t)a1);
+                        e1 = gload(cs, base + (uint32_t)b1 + 1);
                     }
                     scan_direct(s0, e0, s1, e1);
                 }
@@ -1334,12 +1398,12 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                             if (!p0 && !p1) continue;
                             const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
                             const uint32_t a0 = (uint32_t)(p0 ? rp.x : rp.z), b0 = (uint32_t)(p0 ? rp.y : rp.w);
-                            pos = cs[base + a0];
-                            end = cs[base + b0 + 1];
+                            pos = gload(cs, base + a0);
+                            end = gload(cs, base + b0 + 1);
                             if (p0 && p1)
                             {
-                                ps = cs[base + (uint32_t)rp.z];
-                                pe = cs[base + (uint32_t)rp.w + 1];
+                                ps = gload(cs, base + (uint32_t)rp.z);
+                                pe = gload(cs, base + (uint32_t)rp.w + 1);
                             }
                             break;
                         }
@@ -1349,7 +1413,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                 {
                     float4 p[kScanUnroll];
 #pragma unroll
-                    for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = pts[min(pos + u, end - 1)];
+                    for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = gload(pts, min(pos + u, end - 1));
 #pragma unroll
                     for (uint32_t u = 0; u < kScanUnroll; ++u)
                         if (pos + u < end) consider(p[u], pos + u);
@@ -1903,7 +1967,7 @@ hipError_t launch_search_mb(const SearchArgs& a, size_t lds, bool gh, hipStream_
     constexpr int BLOCK = WAVES * 64;
     const uint32_t nb = (a.n_query + BLOCK - 1) / BLOCK;
     if (nb == 0) return hipSuccess;
-    if (a.heap_tail && (gh || !a.gheap_keys)) return hipErrorInvalidValue;
+    if (a.heap_tail && (gh || !a.gheap_keys || (a.heap_tail == 2 && a.tail_pairs == 0))) return hipErrorInvalidValue;
     if (gh)
         k_search<MODE, WAVES, kHeapGlobal, BUDGET><<<nb, BLOCK, (size_t)WAVES * kStage * sizeof(float4), s>>>(a);
     else if (a.heap_tail)

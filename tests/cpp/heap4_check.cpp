@@ -4,7 +4,8 @@
 //
 // Many "lanes" share one LDS-like buffer (pairs of all lanes of a block
 // interleaved) and one global-tail buffer (pairs of all lanes of the launch
-// interleaved), exactly as the kernel lays them out.  Both buffers start
+// interleaved, or the pairs of each wave's 64 lanes interleaved and the
+// waves' tails one after another), exactly as the kernel lays them out.  Both buffers start
 // filled with poison (small, medium and huge keys), so a read of a node the
 // heap has not written shows up as a wrong result.  Each lane offers a random
 // number of random keys (often fewer than k, so partially filled heaps are
@@ -25,6 +26,10 @@ struct ulonglong2
 };
 static ulonglong2 make_ulonglong2(uint64_t a, uint64_t b) { return {a, b}; }
 using std::min;
+// host forms of the kernel's global-memory accessors
+static uint64_t gload(const uint64_t* p, size_t i) { return p[i]; }
+static ulonglong2 gload2(const uint64_t* p) { return {p[0], p[1]}; }
+static void gstore(uint64_t* p, size_t i, uint64_t v) { p[i] = v; }
 
 // @HEAP4@
 
@@ -38,7 +43,10 @@ static uint64_t poison(size_t q)
     }
 }
 
-template <bool TAIL>
+// WAVE_SLOTS: the tail of lane l of wave w (64 lanes) at pairs
+// ((w * tail_pairs + p) * 64 + l) * 2, as with one tail per hardware wave
+// slot; otherwise pair p of lane t at (p * lanes + t) * 2.
+template <bool TAIL, bool WAVE_SLOTS = false>
 static int check(std::mt19937_64& rng)
 {
     constexpr uint64_t kGuard = 0x5a5a5a5a5a5a5a5aull;
@@ -66,8 +74,16 @@ static int check(std::mt19937_64& rng)
             heap[t].keys = lds.data() + (size_t)blk * lds_groups * 4 * kBlock + th * 2;
             heap[t].pstride = 2 * kBlock;
             heap[t].k = k;
-            heap[t].gkeys = glob.data() + (size_t)t * 2;
-            heap[t].gpstride = 2 * kLanes;
+            if (WAVE_SLOTS)
+            {
+                heap[t].gkeys = glob.data() + (((size_t)(t / 64) * tail_pairs) * 64 + t % 64) * 2;
+                heap[t].gpstride = 128;
+            }
+            else
+            {
+                heap[t].gkeys = glob.data() + (size_t)t * 2;
+                heap[t].gpstride = 2 * kLanes;
+            }
             heap[t].init(sentinel);
         }
         const int n = (int)(rng() % 300);
@@ -91,7 +107,9 @@ static int check(std::mt19937_64& rng)
             for (uint32_t i = 0; i < k; ++i)
                 if (heap[t].key_at(i) != want[i])
                 {
-                    if (bad < 10) std::printf("mismatch: tail=%d k=%u lane=%u node=%u\n", (int)TAIL, k, t, i);
+                    if (bad < 10)
+                        std::printf("mismatch: tail=%d slots=%d k=%u lane=%u node=%u\n", (int)TAIL, (int)WAVE_SLOTS,
+                                    k, t, i);
                     ++bad;
                     break;
                 }
@@ -131,7 +149,7 @@ static int check_tail_offsets()
 int main()
 {
     std::mt19937_64 rng(7);
-    const int bad = check<false>(rng) + check<true>(rng) + check_tail_offsets();
+    const int bad = check<false>(rng) + check<true>(rng) + check<true, true>(rng) + check_tail_offsets();
     std::printf("heap4 check: %d mismatches\n", bad);
     return bad != 0;
 }

@@ -3,7 +3,8 @@
 The Heap4 struct is copied out of point_geometric_features_amd/csrc/
 pgeof_search.hip into tests/cpp/heap4_check.cpp, compiled with the host
 compiler (with address and undefined-behaviour checks) and run: for every k
-up to 130, with and without the global tail for k > 21, partially and fully
+up to 130, with and without the global tail for k > 21 (laid out per lane
+of the launch or per hardware wave slot), partially and fully
 filled heaps over poisoned memory must come out as the k smallest keys in
 order (see the .cpp file for the details).
 """

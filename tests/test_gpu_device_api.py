@@ -5,7 +5,9 @@ benchmark times, against the oracle and the host API.
   README.md:136-141): neighbours bit-exact vs the oracle's KD-tree, nn_ptr =
   arange * k, features bit-equal to the two-call path and within the bar of
   the FP64 oracle; k = 22, 50 and 100 use the heap's global tail, k = 50 is
-  the 10 M-point benchmark's second configuration.
+  the 10 M-point benchmark's second configuration; the tails in hardware
+  wave slots equal the tails per launch lane and the all-LDS heap bit for
+  bit.
 * pgeof_radius_csr_device (README.md:158-159): nn_ptr = [0, cumsum(count)],
   nn = idx[idx >= 0] of the oracle's radius search.
 * the device forms of compute_features / _multiscale / _optimal with and
@@ -68,18 +70,47 @@ def test_knn_features_device_lidar(lidar250k, k):
 
 
 def test_knn_features_device_heap_tail_on_off(lidar250k):
-    """k = 50: nodes 21..49 of every heap in global memory, or the whole heap
-    in LDS; the same bits either way."""
+    """k = 50: nodes 21..49 of every heap in global memory, one slot per
+    hardware wave slot (heap_tail 2, the layout of searches of millions of
+    queries) or per lane of the launch (heap_tail 1 at this size), or the
+    whole heap in LDS (0); the same bits every way."""
+    n, k = len(lidar250k), 50
     old = capi.get_tuning("heap_tail")
     try:
         capi.set_tuning("heap_tail", 1)
-        a = _fused(lidar250k, 50)
+        a = _fused(lidar250k, k)
+        per_lane_ws = capi.knn_workspace_size(n, n, k)
+        capi.set_tuning("heap_tail", 2)
+        c = _fused(lidar250k, k)
+        per_slot_ws = capi.knn_workspace_size(n, n, k)
         capi.set_tuning("heap_tail", 0)
-        b = _fused(lidar250k, 50, workspace=False)
+        b = _fused(lidar250k, k, workspace=False)
     finally:
         capi.set_tuning("heap_tail", old)
-    for x, y in zip(a, b):
+    # the wave-slot tails: 65536 slots of 64 lanes, 16 pairs of 16 bytes
+    assert per_slot_ws - per_lane_ws >= 65536 * 64 * 16 * 16 - (n + 256) * 16 * 16
+    for x, y, z in zip(a, b, c):
         np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(x, z)
+
+
+def test_knn_search_wave_slot_heap_tail_vs_oracle():
+    """kNN at k = 64 and 130 with the heap tails in wave slots, uneven
+    density (a dense cluster in a sparse cube): the budgeted and unbudgeted
+    passes both sift into the tail; bit-exact against the oracle."""
+    rng = np.random.default_rng(17)
+    xyz = np.concatenate([rng.random((30_000, 3)) * 40.0,
+                          rng.normal(20.0, 0.3, (30_000, 3))]).astype(np.float32)
+    old = capi.get_tuning("heap_tail")
+    try:
+        capi.set_tuning("heap_tail", 2)
+        res = {k: pgeof.knn_search(xyz, xyz, k) for k in (64, 130)}
+    finally:
+        capi.set_tuning("heap_tail", old)
+    for k, (gi, gd) in res.items():
+        oi, od = O.knn(xyz, xyz, k, kdtree=True)
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gd, od)
 
 
 def test_knn_large_k_binary_global_heap():
