@@ -49,10 +49,21 @@ def test_knn_various_k(k):
     _eq_knn(xyz, xyz, k, kdtree=False)
 
 
-def test_knn_large_k_global_heap():
-    # k too large for the LDS heap budget -> global-memory heap path
+@pytest.mark.parametrize("heap_tail", [1, 0])
+def test_knn_large_k_global_heap(heap_tail):
+    """k = 1200: with heap_tail 1 the 4-ary heap keeps nodes 21..1199 in global
+    memory; with heap_tail 0 the heap cannot fit the LDS budget and the search
+    takes the binary heap held wholly in global memory (heapify after a partial
+    fill, sort in finish)."""
+    from point_geometric_features_amd import capi
+
     xyz = scenes.uniform_cube(1500, seed=5)
-    _eq_knn(xyz, xyz, 1200, kdtree=False)
+    old = capi.get_tuning("heap_tail")
+    try:
+        capi.set_tuning("heap_tail", heap_tail)
+        _eq_knn(xyz, xyz, 1200, kdtree=False)
+    finally:
+        capi.set_tuning("heap_tail", old)
 
 
 def test_knn_k_equals_n():
