@@ -6,12 +6,15 @@ point ``pgeof_knn_features_device`` (results equal the two-call path bit for
 bit).  One step = one pass over the whole cloud (one "cloud" of the per-cloud
 latency).  Prints ONE JSON line.
 
-Other workloads (--workload): lidar10m_k50 (k=50), radius_multiscale_2m
+Other workloads (--workload): uniform10k_k20 (the same path on 10,000 points
+uniform in [0, 1)^3, the reference's README case), lidar10m_k50 (k=50), radius_multiscale_2m
 (radius r=0.5 max 64 -> CSR -> multiscale [10,20,50,100]), optimal_1m
 (precision scene, knn 100 -> optimal k 10..100), features_lidar10m_k20 (the
 CSR feature kernel alone on a precomputed kNN CSR), host_api_lidar10m_k20
 (the Python API on host numpy arrays, PCIe copies included), serving (a
-stream of clouds of 5k..2M points through the serving loop).
+stream of clouds of 5k..2M points through the serving loop), selected_1m
+(compute_features_selected on 1 M float64 points, the README example's
+radius, cap and features at its point density, host arrays).
 
 The default line also carries the serving stream's results (key "serving":
 2,000 requests, p50/p99 latency, open loop vs one request at a time) and,
@@ -453,6 +456,22 @@ def _cpu_multiscale(xyz, r, cap, scales, nn_d, ptr_d, feat_d, sample):
                        f"({t_q:.2f} s, {cores} threads), scaled to all points")}, worst
 
 
+def _selected_err(xyz, r, cap, fids, got, sample):
+    """Per-feature error of compute_features_selected against the oracle's
+    brute-force restatement (same input precision, FP64 PCA) on sampled rows."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(0)
+    s = np.sort(rng.choice(len(xyz), size=min(sample, len(xyz)), replace=False)).astype(np.uint32)
+    ref = O.compute_features_selected(xyz, r, cap, fids, rows=s)
+    g = got[s].astype(np.float64)
+    err = np.abs(g - ref)
+    rel = err / np.maximum(np.abs(ref), 1e-300)
+    return {"rows": int(len(s)), "features": [f.name for f in fids],
+            "max_abs": {f.name: float(f"{err[:, j].max():.3g}") for j, f in enumerate(fids)},
+            "max_rel": {f.name: float(f"{rel[:, j].max():.3g}") for j, f in enumerate(fids)}}
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -488,11 +507,19 @@ def main():
 
     cpu_leg = None  # callable run after the timed steps: (cpu_baseline, feature_err)
     tuned = None  # autotuner report of the workload, when it ran
-    if wl in ("lidar10m_k20", "lidar10m_k50"):
+    if wl in ("lidar10m_k20", "lidar10m_k50", "uniform10k_k20"):
         k = 20 if wl.endswith("k20") else 50
-        n = 10_000_000
-        xyz = scenes.lidar_scene(n, seed=1)
-        cfg.update({"points": n, "k": k, "scene": "synthetic LiDAR 200 m x 200 m (seed 1)",
+        if wl == "uniform10k_k20":
+            # the plain method at the reference's README size: 10,000 points
+            # uniform in [0, 1)^3 (seed 0)
+            n = 10_000
+            xyz = scenes.uniform_cube(n, seed=0)
+            scene = "uniform [0,1)^3 (seed 0)"
+        else:
+            n = 10_000_000
+            xyz = scenes.lidar_scene(n, seed=1)
+            scene = "synthetic LiDAR 200 m x 200 m (seed 1)"
+        cfg.update({"points": n, "k": k, "scene": scene,
                     "path": "knn_search + compute_features, fused, device-resident"})
         x = torch.from_numpy(xyz).to(dev)
         nn = torch.empty(n * k, dtype=torch.int32, device=dev)
@@ -592,6 +619,27 @@ def main():
         kernel = "csr_features_optimal"
         bytes_per_step = (16 + 4 * k + 48) * n
         units = n
+    elif wl == "selected_1m":
+        # the README's compute_features_selected example (uniform float64
+        # points, r = 0.2, max 20 neighbours, Verticality and Curvature) at
+        # its density (10,000 points per unit cube), scaled to 1 M points;
+        # the Python API on host arrays, PCIe copies inside the timed region
+        n, r, cap = 1_000_000, 0.2, 20
+        fids = [pgeof.EFeatureID.Verticality, pgeof.EFeatureID.Curvature]
+        xyz = np.random.default_rng(5).random((n, 3)) * (n / 10_000) ** (1.0 / 3.0)
+        cfg.update({"points": n, "radius": r, "max_knn": cap, "features": [f.name for f in fids],
+                    "dtype": "float64", "path": "pgeof.compute_features_selected on host arrays"})
+        sel_out = {}
+
+        def step():
+            sel_out["f"] = pgeof.compute_features_selected(xyz, r, cap, fids)
+
+        def cpu_leg():
+            return None, _selected_err(xyz, r, cap, fids, sel_out["f"], 2000)
+
+        kernel = "features_selected_f64"
+        bytes_per_step = (24 + 8 * len(fids)) * n  # xyz f64 in, selected features f64 out
+        units = n
     else:
         raise SystemExit(f"unknown workload {wl}")
     cfg["inputs_sha256_16"] = _sha(xyz)
@@ -642,7 +690,7 @@ def main():
                              "candidates": len(tuned["candidates"]), "knobs": tuned["knobs"]}
     # (the plain CSR workload borrows the library's scratch, which a graph
     # cannot hold: the capturable entry points take a workspace)
-    if wl not in ("host_api_lidar10m_k20", "features_lidar10m_k20") and not args.no_graph:
+    if wl not in ("host_api_lidar10m_k20", "features_lidar10m_k20", "selected_1m") and not args.no_graph:
         extra["graph"] = _graph_leg(torch, step, args, units)
     extra["kernels_ms_per_step"] = {kk: round(v[1] / args.steps, 4) for kk, v in sorted(prof.items())}
     extra["kernel_launches_per_step"] = {kk: round(v[0] / args.steps, 2) for kk, v in sorted(prof.items())}

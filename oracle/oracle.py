@@ -55,12 +55,14 @@ def lib() -> ctypes.CDLL:
         L.oracle_knn.argtypes = [P, i64, P, i64, u32, i32, P, P, i32]
         L.oracle_radius.argtypes = [P, i64, P, i64, ctypes.c_float, u32, i32, P, P, P, i32]
         L.oracle_compute_features_selected.argtypes = [P, i32, i64, ctypes.c_double, u32, P, i64, P]
+        L.oracle_compute_features_selected_rows.argtypes = [P, i32, i64, ctypes.c_double, u32, P, i64, P, i64, P]
         L.oracle_knn_features_timed.argtypes = [P, i64, P, i64, u32, i32, P, P,
                                                 ctypes.POINTER(ctypes.c_double),
                                                 ctypes.POINTER(ctypes.c_double)]
         for f in ("oracle_compute_features", "oracle_compute_features_multiscale",
                   "oracle_compute_features_optimal", "oracle_knn", "oracle_radius",
-                  "oracle_compute_features_selected", "oracle_knn_features_timed"):
+                  "oracle_compute_features_selected", "oracle_compute_features_selected_rows",
+                  "oracle_knn_features_timed"):
             getattr(L, f).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -165,13 +167,22 @@ def radius(data, query, r, max_knn, kdtree=True, n_threads=None):
     return idx, d, cnt
 
 
-def compute_features_selected(xyz, r, max_knn, fids) -> np.ndarray:
+def compute_features_selected(xyz, r, max_knn, fids, rows=None) -> np.ndarray:
+    """include/pgeof.hpp:325-375 (brute-force radius search in the input
+    precision); for the given rows only when `rows` is set."""
     is64 = np.asarray(xyz).dtype == np.float64
     x = np.ascontiguousarray(xyz, dtype=np.float64 if is64 else np.float32)
     f = np.ascontiguousarray([int(v) for v in fids], dtype=np.int32)
-    out = np.zeros((len(x), len(f)))
-    lib().oracle_compute_features_selected(_p(x), int(is64), len(x), float(r), int(max_knn), _p(f), len(f),
-                                           _p(out))
+    if rows is None:
+        out = np.zeros((len(x), len(f)))
+        lib().oracle_compute_features_selected(_p(x), int(is64), len(x), float(r), int(max_knn), _p(f), len(f),
+                                               _p(out))
+        return out
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    assert rows.size == 0 or int(rows.max()) < len(x)
+    out = np.zeros((len(rows), len(f)))
+    lib().oracle_compute_features_selected_rows(_p(x), int(is64), len(x), float(r), int(max_knn), _p(f), len(f),
+                                                _p(rows), len(rows), _p(out))
     return out
 
 

@@ -980,61 +980,79 @@ static double selected_value(const pca64_t *p, int fid, int is_f64)
     }
 }
 
-int oracle_compute_features_selected(const void *xyz, int is_f64, int64_t n, double radius, uint32_t max_knn,
-                                     const int32_t *fids, int64_t n_sel, double *out)
+/* One row of compute_features_selected: brute-force radius search of point
+ * i over all n points in the input precision, the min(found, max_knn)
+ * nearest hits (ties by index), rows with fewer than 2 hits stay 0. */
+static void selected_row(const void *xyz, int is_f64, int64_t n, double radius, uint32_t max_knn, const int32_t *fids,
+                         int64_t n_sel, int64_t i, double *hd, uint32_t *hi, double *od, uint32_t *oi, double *row)
 {
-    memset(out, 0, sizeof(double) * (size_t)n * (size_t)(n_sel > 0 ? n_sel : 0));
-    double *hd = (double *)malloc(sizeof(double) * (size_t)(max_knn + 1));
-    uint32_t *hi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(max_knn + 1));
-    double *od = (double *)malloc(sizeof(double) * (size_t)(max_knn + 1));
-    uint32_t *oi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(max_knn + 1));
     const float *xf = (const float *)xyz;
     const double *xd = (const double *)xyz;
     float r2f = (float)radius * (float)radius;
     double r2d = radius * radius;
-    for (int64_t i = 0; i < n; ++i)
+    bheap_t h = {hd, hi, max_knn, 0};
+    int64_t found = 0;
+    for (int64_t j = 0; j < n; ++j)
     {
-        bheap_t h = {hd, hi, max_knn, 0};
-        int64_t found = 0;
-        for (int64_t j = 0; j < n; ++j)
-        {
-            int in;
-            double d;
-            if (is_f64)
-            {
-                d = sqdist_f64(xd + (size_t)i * 3, xd + (size_t)j * 3);
-                in = d < r2d;
-            }
-            else
-            {
-                float df = sqdist_f32(xf + (size_t)i * 3, xf + (size_t)j * 3);
-                d = (double)df;
-                in = df < r2f;
-            }
-            if (in)
-            {
-                found++;
-                if (max_knn > 0) bheap_push(&h, d, (uint32_t)j);
-            }
-        }
-        if (found < 2) continue;
-        int64_t m = h.n;
-        if (m < 1) continue;
-        bheap_drain(&h, od, oi);
-        double cov[6];
+        int in;
+        double d;
         if (is_f64)
-            cov_two_pass_f64(xd, oi, (size_t)m, cov);
+        {
+            d = sqdist_f64(xd + (size_t)i * 3, xd + (size_t)j * 3);
+            in = d < r2d;
+        }
         else
-            cov_two_pass(xf, oi, (size_t)m, cov);
-        pca64_t p;
-        pca_from_cov(cov, &p);
-        for (int64_t s = 0; s < n_sel; ++s) out[(size_t)i * (size_t)n_sel + (size_t)s] = selected_value(&p, fids[s], is_f64);
+        {
+            float df = sqdist_f32(xf + (size_t)i * 3, xf + (size_t)j * 3);
+            d = (double)df;
+            in = df < r2f;
+        }
+        if (in)
+        {
+            found++;
+            if (max_knn > 0) bheap_push(&h, d, (uint32_t)j);
+        }
     }
+    if (found < 2) return;
+    int64_t m = h.n;
+    if (m < 1) return;
+    bheap_drain(&h, od, oi);
+    double cov[6];
+    if (is_f64)
+        cov_two_pass_f64(xd, oi, (size_t)m, cov);
+    else
+        cov_two_pass(xf, oi, (size_t)m, cov);
+    pca64_t p;
+    pca_from_cov(cov, &p);
+    for (int64_t s = 0; s < n_sel; ++s) row[s] = selected_value(&p, fids[s], is_f64);
+}
+
+/* compute_features_selected for the rows listed in `rows` (all rows when
+ * rows is NULL: n_rows = n), out (n_rows, n_sel) zero-initialised here. */
+int oracle_compute_features_selected_rows(const void *xyz, int is_f64, int64_t n, double radius, uint32_t max_knn,
+                                          const int32_t *fids, int64_t n_sel, const uint32_t *rows, int64_t n_rows,
+                                          double *out)
+{
+    const size_t ns = (size_t)(n_sel > 0 ? n_sel : 0);
+    memset(out, 0, sizeof(double) * (size_t)n_rows * ns);
+    double *hd = (double *)malloc(sizeof(double) * (size_t)(max_knn + 1));
+    uint32_t *hi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(max_knn + 1));
+    double *od = (double *)malloc(sizeof(double) * (size_t)(max_knn + 1));
+    uint32_t *oi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(max_knn + 1));
+    for (int64_t r = 0; r < n_rows; ++r)
+        selected_row(xyz, is_f64, n, radius, max_knn, fids, n_sel, rows ? (int64_t)rows[r] : r, hd, hi, od, oi,
+                     out + (size_t)r * ns);
     free(hd);
     free(hi);
     free(od);
     free(oi);
     return 0;
+}
+
+int oracle_compute_features_selected(const void *xyz, int is_f64, int64_t n, double radius, uint32_t max_knn,
+                                     const int32_t *fids, int64_t n_sel, double *out)
+{
+    return oracle_compute_features_selected_rows(xyz, is_f64, n, radius, max_knn, fids, n_sel, NULL, n, out);
 }
 
 /* ------------------------------------------------------------------------ */
