@@ -283,3 +283,18 @@ def test_selected_verticality_and_eigentropy_closed_forms():
         got = O.compute_features_selected(pts.astype(dt), 100.0, 8, list(want))
         for j, (fid, w) in enumerate(want.items()):
             np.testing.assert_allclose(got[:, j], w, atol=tol, err_msg=f"feature {fid} {dt.__name__}")
+
+
+def test_selected_rows_equal_the_full_computation():
+    """The oracle's row-sampled compute_features_selected (used by the bench's
+    error check) gives the rows of the full computation bit for bit, in both
+    precisions, including rows with fewer than 2 hits (zeros)."""
+    rng = np.random.default_rng(61)
+    xyz = rng.random((1500, 3)) * [1.0, 1.0, 0.3]
+    fids = [12, 10, 13, 0, 6]
+    rows = np.array([0, 7, 1499, 700, 7, 31], dtype=np.uint32)
+    for dt in (np.float64, np.float32):
+        x = xyz.astype(dt)
+        full = O.compute_features_selected(x, 0.08, 16, fids)
+        assert (full == 0).all(axis=1).any() and not (full == 0).all()
+        np.testing.assert_array_equal(O.compute_features_selected(x, 0.08, 16, fids, rows=rows), full[rows])
