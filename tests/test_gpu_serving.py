@@ -1,10 +1,14 @@
 """Serving loop: every request's result is the same alone and in the mix
-(concurrent slots, uneven sizes), and equals the direct API."""
+(concurrent slots, uneven sizes), equals the direct API, and matches the
+oracle (neighbours bit for bit, features within the parity bar of
+tests/parity.py against the FP64 restatement)."""
 import numpy as np
 import pytest
 
 import point_geometric_features_amd as pgeof
+from oracle import oracle as O
 from point_geometric_features_amd import capi, scenes
+from tests.parity import assert_features_close
 
 pytestmark = pytest.mark.gpu
 
@@ -48,6 +52,13 @@ def test_serving_alone_equals_in_the_mix():
         np.testing.assert_array_equal(idx.ravel(), mixed[i][1])
         nn, ptr = scenes.knn_csr(idx)
         np.testing.assert_array_equal(pgeof.compute_features(c, nn, ptr), mixed[i][0])
+        # and the oracle: kNN rows (include/nn_search.hpp:31-67) and FP64
+        # features of those rows (include/pgeof.hpp:75-117)
+        oi, _ = O.knn(c, c, k, kdtree=True)
+        np.testing.assert_array_equal(mixed[i][1].reshape(-1, k), oi)
+        onn, optr = scenes.knn_csr(oi)
+        assert_features_close(mixed[i][0], O.compute_features(c, onn, optr), O.eigvals_csr_np(c, onn, optr),
+                              f"serving request {i} ({len(c)} points)")
 
 
 def test_serving_admission_limits():
