@@ -1024,6 +1024,10 @@ __device__ __forceinline__ Level uniform_level(const LevelTable* tab, int l)
 #define PGEOF_SCAN_UNROLL 4
 #endif
 constexpr uint32_t kScanUnroll = PGEOF_SCAN_UNROLL;  // candidates loaded together per lane
+// unbudgeted pass (per-lane cursor): candidates loaded together per lane
+// (10 M points, k = 20: 4 -> 3.5 ms, 6 -> 3.35, 8 -> 3.35 with three times
+// the spill of 6; k = 50: 8.4, 7.9-8.2, 7.8-8.0)
+constexpr uint32_t kCursorUnroll = 6;
 // budgeted kNN pass: rows of a plane scanned as one stream per lane (3 rows:
 // 6.3 -> 5.8 ms per 10 M queries at k = 20; 2 rows 6.0, 4 rows 5.9)
 constexpr int kRowsPerStream = 3;
@@ -1291,11 +1295,11 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
         {
             // budgeted pass: the lanes walk their rows in step (nearest
             // first), a row's setup shared by the whole wave
-            const int zc = a.near_first ? cz : B.z0, yc = a.near_first ? cy : B.y0;
-            for (int iz = 0; iz < 2 * (B.z1 - B.z0 + 1); ++iz)
+            const int zc = a.near_first ? min(max(cz, B.z0), B.z1) : B.z0;
+            const int yc = a.near_first ? min(max(cy, B.y0), B.y1) : B.y0;
+            for (int iz = 0; iz <= B.z1 - B.z0; ++iz)
             {
-                const int z = near_index(iz, zc, B.z0, B.z1);
-                if (z < B.z0 || z > B.z1) continue;
+                const int z = nth_near(iz, zc, B.z0, B.z1);
                 double dz2 = 0.0;
                 if (tighten)
                 {
@@ -1308,15 +1312,15 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                 // busiest lane's sum of those pieces instead of each piece's
                 // busiest lane, and the rows' cell starts load together
                 constexpr int kR = kRowsPerStream;
-                for (int iy = 0; iy < 2 * (B.y1 - B.y0 + 1); iy += kR)
+                for (int iy = 0; iy <= B.y1 - B.y0; iy += kR)
                 {
                     uint32_t rs[2 * kR], rl[2 * kR];
 #pragma unroll
                     for (int h2 = 0; h2 < kR; ++h2)
                     {
-                        const int y = near_index(iy + h2, yc, B.y0, B.y1);
                         rs[2 * h2] = rl[2 * h2] = rs[2 * h2 + 1] = rl[2 * h2 + 1] = 0;
-                        if (y < B.y0 || y > B.y1) continue;
+                        if (iy + h2 > B.y1 - B.y0) continue;
+                        const int y = nth_near(iy + h2, yc, B.y0, B.y1);
                         const int4 rp = row_pieces(B, C, hasC, tighten, dz2, y, z);
                         const uint32_t base = ((uint32_t)z * gy + (uint32_t)y) * gx;
                         if (rp.x <= rp.y)
@@ -1433,13 +1437,13 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                 }
                 if (pos < end)
                 {
-                    float4 p[kScanUnroll];
+                    float4 p[kCursorUnroll];
 #pragma unroll
-                    for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = gload(pts, min(pos + u, end - 1));
+                    for (uint32_t u = 0; u < kCursorUnroll; ++u) p[u] = gload(pts, min(pos + u, end - 1));
 #pragma unroll
-                    for (uint32_t u = 0; u < kScanUnroll; ++u)
+                    for (uint32_t u = 0; u < kCursorUnroll; ++u)
                         if (pos + u < end) consider(p[u]);
-                    pos = min(pos + kScanUnroll, end);
+                    pos = min(pos + kCursorUnroll, end);
                 }
             }
         }
