@@ -253,9 +253,9 @@ void pgeof_server_destroy(pgeof_server* server);
  * hardware wave slot, 128 MiB per 4 tail nodes, smaller ones per query;
  * 2: per hardware wave slot at any size; 0: the whole heap in LDS while it
  * fits), "staging"
- * (wave-shared LDS staging of cell rows: 0 never, 1 radius search and,
- * for k >= 32, the budgeted kNN pass, 2 radius search and the budgeted kNN
- * pass at any k), "fuse_multiscale" (pgeof_radius_multiscale_device: 1 the
+ * (wave-shared LDS staging of cell rows: 0 never (default: each lane streams
+ * its rows' candidates from L2), 1 radius search and, for k >= 32, the
+ * budgeted kNN pass, 2 radius search and the budgeted kNN pass at any k), "fuse_multiscale" (pgeof_radius_multiscale_device: 1 the
  * features in the search kernel, 0 in a CSR kernel after it).
  * Knobs change speed only, never results. Values must be finite and >= 0. */
 int pgeof_set_tuning(const char* key, double value);

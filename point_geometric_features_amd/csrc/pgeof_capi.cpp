@@ -65,7 +65,7 @@ struct Tuning
     double grid_level_ratio = 2.0;      // cell edge ratio between consecutive pyramid levels (2..8)
     double csr_order = 1.0;             // CSR features: visit large inputs' rows in spatial order
     double heap_tail = 1.0;             // search heaps: nodes past 20 in global memory (k > 21); 2: per wave slot always
-    double staging = 1.0;               // LDS staging of cell rows: 0 never, 1 radius + budgeted kNN at k >= 32, 2 always
+    double staging = 0.0;               // LDS staging of cell rows: 0 never, 1 radius + budgeted kNN at k >= 32, 2 always
     double fuse_multiscale = 0.0;       // radius + multiscale: features in the search kernel (1) or a CSR kernel after it (0)
 };
 Tuning g_tune;
@@ -1588,7 +1588,7 @@ int pgeof_autotune(const char* workload, const float* xyz, int64_t n, uint32_t k
         for (double cpp : {4.0, 8.0, 16.0})
             for (double occ : {0.25, 0.35, 0.5})
                 for (double lds : {65536.0, 32768.0})
-                    for (double st : {1.0, 2.0})
+                    for (double st : {0.0, 2.0})
                     {
                         Tuning t = base;
                         t.grid_cells_per_point = cpp;

@@ -929,12 +929,14 @@ __device__ __forceinline__ double axis_gap(double v, int c, double lo, double h,
 // k points (kNN; radius searches use level 0), so its row count stays small
 // whatever the local density.  Rows are then scanned in one of three ways,
 // chosen per wavefront and pass:
-//   - cooperative (radius searches; budgeted kNN pass for k >= 32): when all
+//   - cooperative (with the staging knob: radius searches, and the budgeted
+//     kNN pass for k >= 32 or always): when all
 //     lanes of the wave use the same level and the union of their rows is
 //     compact, the wave walks that union in lockstep, stages each row's
 //     point slice into LDS with coalesced loads and every lane scans its own
 //     sub-range from LDS;
-//   - per lane in step (budgeted kNN pass): each lane walks its own rows,
+//   - per lane in step (budgeted kNN pass, radius searches without
+//     staging): each lane walks its own rows,
 //     the wave opening rows r..r+2 of a plane of every lane together and
 //     each lane scanning their pieces as one stream of candidates;
 //   - per lane with a cursor (unbudgeted pass): each lane opens its next row
@@ -1291,10 +1293,10 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                 }
             }
         }
-        else if (has && BUDGET)
+        else if (has && (BUDGET || RADIUS))
         {
-            // budgeted pass: the lanes walk their rows in step (nearest
-            // first), a row's setup shared by the whole wave
+            // budgeted kNN pass and radius searches: the lanes walk their
+            // rows in step (nearest first), a row's setup shared by the wave
             const int zc = a.near_first ? min(max(cz, B.z0), B.z1) : B.z0;
             const int yc = a.near_first ? min(max(cy, B.y0), B.y1) : B.y0;
             for (int iz = 0; iz <= B.z1 - B.z0; ++iz)

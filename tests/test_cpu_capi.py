@@ -96,7 +96,7 @@ def test_tuning_roundtrip():
 _KNOBS = {
     "knn_occupancy": 0.35, "knn_cell_points_max": 25.0, "grid_cells_per_point": 8.0, "lds_budget": 65536.0,
     "radius_cell_factor": 0.5, "grid_levels": 3.0, "near_first": 1.0, "budget_cells": 2.0, "budget_shells": 1.0,
-    "budget_levels": 1.0, "grid_level_ratio": 2.0, "csr_order": 1.0, "heap_tail": 1.0, "staging": 1.0,
+    "budget_levels": 1.0, "grid_level_ratio": 2.0, "csr_order": 1.0, "heap_tail": 1.0, "staging": 0.0,
     "fuse_multiscale": 0.0,
 }
 
