@@ -944,7 +944,7 @@ int selected_host(const T* xyz, int64_t n, T search_radius, uint32_t max_knn, co
     int rc = ctx_prepare(&s);
     if (rc) return rc;
     const HeapPlan hp = heap_plan_bytes(cap, sizeof(T) + 4);
-    const size_t b_x = al((size_t)n * 3 * sizeof(T)), b_xf = f64 ? al((size_t)n * 12) : 0,
+    const size_t b_x = al((size_t)n * 3 * sizeof(T)), b_xf = f64 ? al((size_t)n * 12) + al((size_t)n * 24) : 0,
                  b_o = al((size_t)n * (size_t)n_selected * sizeof(T)), b_f = al((size_t)n_selected * 4),
                  b_h = hp.global ? al((size_t)n * cap * sizeof(T)) + al((size_t)n * cap * 4) : 0;
     const size_t b_ws = search_ws_bytes((uint64_t)n, (uint64_t)n, 1, false);
@@ -953,6 +953,7 @@ int selected_host(const T* xyz, int64_t n, T search_radius, uint32_t max_knn, co
     Arena ar{t_ctx.dbuf, t_ctx.dbytes, 0};
     T* d_x = ar.take<T>((size_t)n * 3);
     float* d_xf = f64 ? ar.take<float>((size_t)n * 3) : reinterpret_cast<float*>(d_x);
+    double* d_xs = f64 ? ar.take<double>((size_t)n * 3) : nullptr;  // sorted copy of the doubles
     T* d_out = ar.take<T>((size_t)n * (size_t)n_selected);
     int32_t* d_f = ar.take<int32_t>((size_t)n_selected);
     void* d_hk = hp.global ? (void*)ar.take<T>((size_t)n * cap) : nullptr;
@@ -1001,6 +1002,11 @@ int selected_host(const T* xyz, int64_t n, T search_radius, uint32_t max_knn, co
     a.out = d_out;
     a.gheap_keys = d_hk;
     a.gheap_idx = d_hi;
+    if (f64)
+    {
+        HIP_TRY(launch_sorted_copy_f64(L0.pts, reinterpret_cast<const double*>(d_x), (uint32_t)n, d_xs, s));
+        a.xs = d_xs;
+    }
     HIP_TRY(launch_selected(f64, a, hp.block, hp.lds, hp.global, s));
     HIP_TRY(hipMemcpyAsync(out, d_out, (size_t)n * (size_t)n_selected * sizeof(T), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

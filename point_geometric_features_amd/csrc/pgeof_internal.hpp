@@ -197,6 +197,7 @@ struct SelectedArgs
 {
     GridView grid;       // grid over float32 copies of the points, self queries
     const void* xyz;     // original coordinates (float or double), (n, 3)
+    const double* xs;    // double input: the coordinates in the grid's sorted order, (n, 3); float: nullptr
     uint32_t n;
     uint32_t cap;        // heap capacity: max(max_knn, 2)
     uint32_t max_knn;
@@ -210,6 +211,7 @@ struct SelectedArgs
 };
 hipError_t launch_selected(bool f64, const SelectedArgs& a, int block, size_t lds, bool global_heap, hipStream_t s);
 hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s);
+hipError_t launch_sorted_copy_f64(const float4* pts, const double* xyz, uint32_t n, double* xs, hipStream_t s);
 
 enum SearchMode
 {

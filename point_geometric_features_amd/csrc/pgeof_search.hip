@@ -1893,9 +1893,29 @@ __global__ __launch_bounds__(BLOCK) void k_selected(SelectedArgs a)
         heap.idx[(size_t)j * heap.stride] = 0xFFFFFFFFu;
     }
     const T* __restrict__ X = reinterpret_cast<const T*>(a.xyz);
-    const float4 qf = a.grid.pts[t];
+    const float4* __restrict__ pts = a.grid.pts;
+    // candidate coordinates in the input precision, read in the grid's
+    // sorted order: float input is the sorted float4 array itself, double
+    // input a sorted copy (a.xs); the original index comes from pts[j].w
+    const double* __restrict__ xs = a.xs;
+    auto coords = [&](uint32_t j, const float4& pj, T c[3]) {
+        if constexpr (sizeof(T) == 8)
+        {
+            c[0] = xs[(size_t)j * 3 + 0];
+            c[1] = xs[(size_t)j * 3 + 1];
+            c[2] = xs[(size_t)j * 3 + 2];
+        }
+        else
+        {
+            c[0] = pj.x;
+            c[1] = pj.y;
+            c[2] = pj.z;
+        }
+    };
+    const float4 qf = pts[t];
     const uint32_t qi = __float_as_uint(qf.w);
-    const T q[3] = {X[(size_t)qi * 3 + 0], X[(size_t)qi * 3 + 1], X[(size_t)qi * 3 + 2]};
+    T q[3];
+    coords(t, qf, q);
     T worst_k = r2;
     uint32_t worst_i = 0xFFFFFFFFu;
     const GridParams& gp = a.grid.p;
@@ -1914,16 +1934,30 @@ __global__ __launch_bounds__(BLOCK) void k_selected(SelectedArgs a)
             {
                 const uint32_t base = ((uint32_t)z * (uint32_t)gp.g[1] + (uint32_t)y) * (uint32_t)gp.g[0];
                 const uint32_t s = a.grid.cell_start[base + (uint32_t)x0], e = a.grid.cell_start[base + (uint32_t)x1 + 1];
-                for (uint32_t j = s; j < e; ++j)
+                // four candidates loaded together, then considered in order
+                for (uint32_t j = s; j < e; j += 4)
                 {
-                    const uint32_t pi = __float_as_uint(a.grid.pts[j].w);
-                    const T p[3] = {X[(size_t)pi * 3 + 0], X[(size_t)pi * 3 + 1], X[(size_t)pi * 3 + 2]};
-                    const T d = sqdist_t<T>(q, p);
-                    if (d < r2 && PairHeap<T>::less(d, pi, worst_k, worst_i))
+                    float4 pj[4];
+                    T pc[4][3];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
                     {
-                        heap.sift_root(d, pi, cap);
-                        worst_k = heap.key[0];
-                        worst_i = heap.idx[0];
+                        const uint32_t jj = min(j + u, e - 1);
+                        pj[u] = pts[jj];
+                        coords(jj, pj[u], pc[u]);
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                    {
+                        if (j + u >= e) break;
+                        const uint32_t pi = __float_as_uint(pj[u].w);
+                        const T d = sqdist_t<T>(q, pc[u]);
+                        if (d < r2 && PairHeap<T>::less(d, pi, worst_k, worst_i))
+                        {
+                            heap.sift_root(d, pi, cap);
+                            worst_k = heap.key[0];
+                            worst_i = heap.idx[0];
+                        }
                     }
                 }
             }
@@ -1978,6 +2012,19 @@ hipError_t launch_selected_t(const SelectedArgs& a, int block, size_t lds, bool 
         case 128: return launch_selected_b<T, 128>(a, lds, gh, s);
         default: return launch_selected_b<T, 256>(a, lds, gh, s);
     }
+}
+
+// xs[j] = xyz[original index of sorted point j] (the grid's order), doubles
+__global__ __launch_bounds__(256) void k_sorted_copy_f64(const float4* __restrict__ pts,
+                                                         const double* __restrict__ xyz, uint32_t n,
+                                                         double* __restrict__ xs)
+{
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t pi = __float_as_uint(pts[j].w);
+    xs[(size_t)j * 3 + 0] = xyz[(size_t)pi * 3 + 0];
+    xs[(size_t)j * 3 + 1] = xyz[(size_t)pi * 3 + 1];
+    xs[(size_t)j * 3 + 2] = xyz[(size_t)pi * 3 + 2];
 }
 
 __global__ __launch_bounds__(256) void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, uint64_t n)
@@ -2321,6 +2368,13 @@ hipError_t launch_selected(bool f64, const SelectedArgs& a, int block, size_t ld
 {
     ProfScope ps(f64 ? "features_selected_f64" : "features_selected_f32", s);
     return f64 ? launch_selected_t<double>(a, block, lds, gh, s) : launch_selected_t<float>(a, block, lds, gh, s);
+}
+
+hipError_t launch_sorted_copy_f64(const float4* pts, const double* xyz, uint32_t n, double* xs, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_sorted_copy_f64<<<(n + 255) / 256, 256, 0, s>>>(pts, xyz, n, xs);
+    return hipGetLastError();
 }
 
 hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s)

@@ -232,7 +232,7 @@ _KNOB_SETS = [
     {"lds_budget": 16384},
     {"lds_budget": 4096},  # heaps in global memory
     {"heap_tail": 0},  # radius cap 24: the whole heap in LDS instead of nodes 21..23 in global memory
-    {"staging": 0},
+    {"staging": 1},  # radius searches staged, the budgeted kNN pass at k >= 32
     {"staging": 2},
 ]
 
@@ -285,3 +285,31 @@ def test_host_search_after_outputs_of_odd_length():
     oi, od, _ = O.radius(data, q, 0.4, 9, kdtree=True)
     np.testing.assert_array_equal(ri, oi)
     np.testing.assert_array_equal(rd, od)
+
+
+@pytest.mark.parametrize("staging", [0, 1, 2])
+def test_knn_k50_row_staging_changes_speed_only(staging):
+    """k = 50 (heap tails, the budgeted pass staged in LDS with staging 1 and
+    2, per-lane row streams with 0): the oracle's neighbours bit for bit, and
+    the fused features equal compute_features on that CSR."""
+    from point_geometric_features_amd import capi
+
+    xyz = scenes.lidar_scene(60_000, seed=31, extent=25.0)
+    old = capi.get_tuning("staging")
+    try:
+        capi.set_tuning("staging", staging)
+        gi, gd = pgeof.knn_search(xyz, xyz, 50)
+        x = torch.from_numpy(xyz).cuda()
+        nn_d = torch.empty(len(xyz) * 50, dtype=torch.int32, device="cuda")
+        ptr_d = torch.empty(len(xyz) + 1, dtype=torch.int32, device="cuda")
+        f_d = torch.empty((len(xyz), 11), dtype=torch.float32, device="cuda")
+        capi.knn_features_device(x, 50, nn_d, ptr_d, f_d)
+        torch.cuda.synchronize()
+    finally:
+        capi.set_tuning("staging", old)
+    oi, od = O.knn(xyz, xyz, 50, kdtree=True)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(nn_d.cpu().numpy().view(np.uint32), gi.reshape(-1))
+    nn, ptr = scenes.knn_csr(gi)
+    np.testing.assert_array_equal(f_d.cpu().numpy(), pgeof.compute_features(xyz, nn, ptr, 50))
