@@ -997,6 +997,7 @@ int selected_host(const T* xyz, int64_t n, T search_radius, uint32_t max_knn, co
     // the grid holds float32 copies: widen the cell range by their rounding
     const double margin = gv.p.h * 1e-6 + (f64 ? 2.4e-7 * maxabs : 0.0);
     a.reach = std::fabs((double)search_radius) + margin;
+    a.margin = margin;
     a.fids = d_f;
     a.n_sel = (uint32_t)n_selected;
     a.out = d_out;

@@ -203,6 +203,7 @@ struct SelectedArgs
     uint32_t max_knn;
     double r2;           // search_radius^2 evaluated in the input precision
     double reach;        // search_radius + margin, for the cell range
+    double margin;       // slack of the float grid copies' rounding, added to any distance bound
     const int32_t* fids;  // EFeatureID list
     uint32_t n_sel;
     void* out;            // (n, n_sel) in the input precision

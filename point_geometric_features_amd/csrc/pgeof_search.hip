@@ -1923,17 +1923,40 @@ __global__ __launch_bounds__(BLOCK) void k_selected(SelectedArgs a)
     if (finite)
     {
         const double R = a.reach;
-        const int x0 = cell_coord((double)qf.x - R, gp.lo[0], gp.inv_h, gp.g[0]);
-        const int x1 = cell_coord((double)qf.x + R, gp.lo[0], gp.inv_h, gp.g[0]);
-        const int y0 = cell_coord((double)qf.y - R, gp.lo[1], gp.inv_h, gp.g[1]);
-        const int y1 = cell_coord((double)qf.y + R, gp.lo[1], gp.inv_h, gp.g[1]);
-        const int z0 = cell_coord((double)qf.z - R, gp.lo[2], gp.inv_h, gp.g[2]);
-        const int z1 = cell_coord((double)qf.z + R, gp.lo[2], gp.inv_h, gp.g[2]);
-        for (int z = z0; z <= z1; ++z)
-            for (int y = y0; y <= y1; ++y)
+        const double fx = qf.x, fy = qf.y, fz = qf.z;
+        const int x0 = cell_coord(fx - R, gp.lo[0], gp.inv_h, gp.g[0]);
+        const int x1 = cell_coord(fx + R, gp.lo[0], gp.inv_h, gp.g[0]);
+        const int y0 = cell_coord(fy - R, gp.lo[1], gp.inv_h, gp.g[1]);
+        const int y1 = cell_coord(fy + R, gp.lo[1], gp.inv_h, gp.g[1]);
+        const int z0 = cell_coord(fz - R, gp.lo[2], gp.inv_h, gp.g[2]);
+        const int z1 = cell_coord(fz + R, gp.lo[2], gp.inv_h, gp.g[2]);
+        const int yc = min(max(cell_coord(fy, gp.lo[1], gp.inv_h, gp.g[1]), y0), y1);
+        const int zc = min(max(cell_coord(fz, gp.lo[2], gp.inv_h, gp.g[2]), z0), z1);
+        // rows nearest first, each cut to the sphere of the current bound (in
+        // the float grid's space: the radius, or the heap's worst distance
+        // once it is full, plus the rounding margin of the float copies)
+        auto bound2 = [&]() -> double {
+            const double b = worst_i == 0xFFFFFFFFu ? R : sqrt((double)worst_k * 1.00001) + a.margin;
+            return (b < R ? b : R) * (b < R ? b : R);
+        };
+        const double gm = gp.h * 1e-6;  // cell boxes widened against rounding of the cell coordinates
+        for (int iz = 0; iz <= z1 - z0; ++iz)
+        {
+            const int z = nth_near(iz, zc, z0, z1);
+            const double dz = axis_gap(fz, z, gp.lo[2], gp.h, gm);
+            if (dz * dz > bound2()) continue;
+            for (int iy = 0; iy <= y1 - y0; ++iy)
             {
+                const int y = nth_near(iy, yc, y0, y1);
+                const double dy = axis_gap(fy, y, gp.lo[1], gp.h, gm);
+                const double rem = bound2() - dz * dz - dy * dy;
+                if (rem < 0.0) continue;
+                const double xr = sqrt(rem) + gm;
+                const int xa = max(x0, cell_coord(fx - xr, gp.lo[0], gp.inv_h, gp.g[0]));
+                const int xb = min(x1, cell_coord(fx + xr, gp.lo[0], gp.inv_h, gp.g[0]));
+                if (xa > xb) continue;
                 const uint32_t base = ((uint32_t)z * (uint32_t)gp.g[1] + (uint32_t)y) * (uint32_t)gp.g[0];
-                const uint32_t s = a.grid.cell_start[base + (uint32_t)x0], e = a.grid.cell_start[base + (uint32_t)x1 + 1];
+                const uint32_t s = a.grid.cell_start[base + (uint32_t)xa], e = a.grid.cell_start[base + (uint32_t)xb + 1];
                 // four candidates loaded together, then considered in order
                 for (uint32_t j = s; j < e; j += 4)
                 {
@@ -1961,6 +1984,7 @@ __global__ __launch_bounds__(BLOCK) void k_selected(SelectedArgs a)
                     }
                 }
             }
+        }
     }
     heap.sort_ascending();
     uint32_t cnt = 0;

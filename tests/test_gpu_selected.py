@@ -72,3 +72,48 @@ def test_selected_type_errors():
         pgeof.compute_features_selected(np.zeros((5, 3), np.int32), 0.1, 4, [F.Linearity])
     with pytest.raises(TypeError):
         pgeof.compute_features_selected(np.zeros((5, 2), np.float32), 0.1, 4, [F.Linearity])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_selected_lidar_rows_vs_oracle(dtype):
+    """A 120k-point LiDAR-like scene (planes, poles, blobs, sparse fringes) at
+    r = 0.3 with max 16 neighbours: the rows of many-hit neighbourhoods go
+    through the kernel's pruning of rows by the heap's current worst
+    distance.  2,000 sampled rows against the oracle's brute-force
+    restatement in the same precision (include/pgeof.hpp:325-375)."""
+    from point_geometric_features_amd import scenes
+
+    xyz = scenes.lidar_scene(120_000, seed=71, extent=20.0).astype(dtype)
+    fids = [F.Verticality, F.Curvature, F.Eigentropy, F.Linearity, F.Surface]
+    got = pgeof.compute_features_selected(xyz, 0.3, 16, fids)
+    rows = np.sort(np.random.default_rng(72).choice(len(xyz), 2000, replace=False)).astype(np.uint32)
+    ref = O.compute_features_selected(xyz, 0.3, 16, fids, rows=rows)
+    rtol, atol = (1e-5, 1e-4) if dtype == np.float32 else (1e-9, 1e-8)
+    # rank-deficient neighbourhoods (a pole seen edge-on, a wall's line of
+    # points) leave Linearity's rounding-level sqrt(lambda_min) unpinned
+    x32 = xyz.astype(np.float32)
+    idx, _, _ = O.radius(x32, x32[rows], 0.3, 16, kdtree=True)
+    nn = idx[idx >= 0].astype(np.uint32)
+    ptr = np.r_[0, (idx >= 0).sum(1).cumsum()].astype(np.uint32)
+    val = O.eigvals_csr_np(x32, nn, ptr)
+    ok = val[:, 0] > 1e-9 * np.maximum(val[:, 2], 1e-300)
+    g = got[rows].astype(np.float64)
+    np.testing.assert_allclose(g[ok], ref[ok], rtol=rtol, atol=atol)
+    np.testing.assert_allclose(g[:, [0, 1, 2, 4]], ref[:, [0, 1, 2, 4]], rtol=max(rtol, 1e-4), atol=atol)
+    assert (g == 0).all(axis=1).sum() == (ref == 0).all(axis=1).sum()
+
+
+def test_selected_lattice_ties_by_index():
+    """A lattice where many neighbours tie on distance: the max_knn kept are
+    the smallest indices among the ties, as the oracle (float and double)."""
+    g = np.stack(np.meshgrid(*[np.arange(14)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(73)
+    g = g[rng.permutation(len(g))]
+    # eigenvalue-only features: the lattice's neighbourhoods have repeated
+    # eigenvalues, whose eigenvectors (the normal) the reference leaves open
+    fids = [F.Linearity, F.Planarity, F.Scattering, F.Curvature, F.Eigentropy]
+    for dtype, tol in ((np.float32, 1e-5), (np.float64, 1e-12)):
+        x = (g * 0.1).astype(dtype)
+        got = pgeof.compute_features_selected(x, 0.15, 9, fids)
+        ref = O.compute_features_selected(x, 0.15, 9, fids)
+        np.testing.assert_allclose(got.astype(np.float64), ref, rtol=tol, atol=tol)
