@@ -1161,10 +1161,36 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
         return (double)w * 1.00001 + 1e-37;
     };
 
-    auto consider = [&](const float4& p) {
+    // POSKEY (budgeted fused pass with heap tails, k > 21): keys carry the
+    // candidate's position in the level's sorted array instead of its
+    // original index, so that the epilogue gathers the k neighbours from the
+    // spatially sorted points rather than from the caller's array in index
+    // order (10 M points, k = 50: budgeted pass 19.2 -> 17.0 ms; at k = 20
+    // the tie bookkeeping costs what the gathers save).  Ties in distance
+    // are then ordered by position, not by index: any exact tie that could
+    // matter (at the k-th entry, or between two entries of the row) sends
+    // the query to the unbudgeted pass, which keys by index.
+    constexpr bool POSKEY = BUDGET && MODE == kModeKnnFeat && HEAP == kHeapTail;
+    bool tie = false;
+    auto consider = [&](const float4& p, uint32_t pos) {
         const float d2 = sqdist32(qxf, qyf, qzf, p.x, p.y, p.z);
-        const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(p.w);
-        if (key < worst) heap.add(key, worst);
+        const uint32_t lo = POSKEY ? pos : __float_as_uint(p.w);
+        const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)lo;
+        if constexpr (POSKEY)
+        {
+            // a candidate with the worst entry's distance: rejected or
+            // replacing it, the choice between the two is a tie
+            const uint32_t wd = (uint32_t)(worst >> 32);
+            tie = tie || __float_as_uint(d2) == wd;
+            if (key < worst)
+            {
+                heap.add(key, worst);
+                // the entry now at the root has the evicted one's distance
+                tie = tie || (wd != 0xFFFFFFFFu && (uint32_t)(worst >> 32) == wd);
+            }
+        }
+        else if (key < worst)
+            heap.add(key, worst);
     };
 
 
@@ -1184,7 +1210,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                 for (uint32_t u = 0; u < kScanUnroll; ++u) p[u] = stage[min(j + u, hi - 1) - c];
 #pragma unroll
                 for (uint32_t u = 0; u < kScanUnroll; ++u)
-                    if (j + u < hi) consider(p[u]);
+                    if (j + u < hi) consider(p[u], j + u);
             }
         }
     };
@@ -1350,6 +1376,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                     for (uint32_t c = 0; c < tot; c += kScanUnroll)
                     {
                         float4 p[kScanUnroll];
+                        uint32_t pp[kScanUnroll];
 #pragma unroll
                         for (uint32_t u = 0; u < kScanUnroll; ++u)
                         {
@@ -1357,11 +1384,12 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                             uint32_t o = off[2 * kR - 1];
 #pragma unroll
                             for (int r = 2 * kR - 2; r >= 0; --r) o = cc < cum[r] ? off[r] : o;
-                            p[u] = gload(pts, cc + o);
+                            pp[u] = cc + o;
+                            p[u] = gload(pts, pp[u]);
                         }
 #pragma unroll
                         for (uint32_t u = 0; u < kScanUnroll; ++u)
-                            if (c + u < tot) consider(p[u]);
+                            if (c + u < tot) consider(p[u], pp[u]);
                     }
                 }
             }
@@ -1444,7 +1472,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                     for (uint32_t u = 0; u < kCursorUnroll; ++u) p[u] = gload(pts, min(pos + u, end - 1));
 #pragma unroll
                     for (uint32_t u = 0; u < kCursorUnroll; ++u)
-                        if (pos + u < end) consider(p[u]);
+                        if (pos + u < end) consider(p[u], pos + u);
                     pos = min(pos + kCursorUnroll, end);
                 }
             }
@@ -1513,6 +1541,23 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
     if (!active) return;
     if (BUDGET)
     {
+        if constexpr (POSKEY)
+        {
+            if (!deferred)
+            {
+                // two entries of the row with one distance: their order (and
+                // with a tie at the k-th, the choice) follows positions here
+                heap.finish(sentinel);
+                uint32_t prev = 0xFFFFFFFFu;
+                for (uint32_t j = 0; j < a.k; ++j)
+                {
+                    const uint32_t d = (uint32_t)(heap.key_at(j) >> 32);
+                    tie = tie || d == prev;
+                    prev = d;
+                }
+                deferred = tie;
+            }
+        }
         a.defer_flag[t] = deferred ? 1u : 0u;
         if (deferred)
         {
@@ -1522,7 +1567,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
             return;
         }
     }
-    heap.finish(sentinel);
+    if constexpr (!POSKEY) heap.finish(sentinel);
     const uint32_t k = a.k;
 
     if (MODE == kModeKnn)
@@ -1633,22 +1678,33 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
             Moments m;
             mom_zero(m);
             const float3* __restrict__ xyz3 = reinterpret_cast<const float3*>(a.data);
+            // neighbour of a key: its coordinates and (bits of) original index,
+            // from the sorted points by position (POSKEY) or from xyz by index
+            auto nbr = [&](uint64_t key) -> float4 {
+                if constexpr (POSKEY)
+                    return gload(pts, (uint32_t)key);
+                else
+                {
+                    const float3 q3 = xyz3[(uint32_t)key];
+                    return make_float4(q3.x, q3.y, q3.z, __uint_as_float((uint32_t)key));
+                }
+            };
             constexpr uint32_t kChunk = 4;
             const bool vec_out =
                 (k & 3u) == 0 && ((reinterpret_cast<uintptr_t>(a.out_idx) | reinterpret_cast<uintptr_t>(a.out_d2)) & 15u) == 0;
             uint64_t kn[kChunk];
-            float3 pn[kChunk];
+            float4 pn[kChunk];
 #pragma unroll
             for (uint32_t u = 0; u < kChunk; ++u)
             {
                 kn[u] = heap.key_at(u < k ? u : 0);
-                pn[u] = xyz3[(uint32_t)kn[u]];
+                pn[u] = nbr(kn[u]);
             }
             const double ox = pn[0].x, oy = pn[0].y, oz = pn[0].z;
             for (uint32_t j0 = 0; j0 < k; j0 += kChunk)
             {
                 uint64_t kc[kChunk];
-                float3 pc[kChunk];
+                float4 pc[kChunk];
 #pragma unroll
                 for (uint32_t u = 0; u < kChunk; ++u)
                 {
@@ -1662,13 +1718,14 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                     {
                         const uint32_t j = j0 + kChunk + u;
                         kn[u] = heap.key_at(j < k ? j : 0);
-                        pn[u] = xyz3[(uint32_t)kn[u]];
+                        pn[u] = nbr(kn[u]);
                     }
                 }
                 if (vec_out)
                 {
                     *reinterpret_cast<uint4*>(oi + j0) =
-                        make_uint4((uint32_t)kc[0], (uint32_t)kc[1], (uint32_t)kc[2], (uint32_t)kc[3]);
+                        make_uint4(__float_as_uint(pc[0].w), __float_as_uint(pc[1].w), __float_as_uint(pc[2].w),
+                                   __float_as_uint(pc[3].w));
                     if (od)
                         *reinterpret_cast<float4*>(od + j0) = make_float4(
                             __uint_as_float((uint32_t)(kc[0] >> 32)), __uint_as_float((uint32_t)(kc[1] >> 32)),
@@ -1680,7 +1737,7 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                     for (uint32_t u = 0; u < kChunk; ++u)
                         if (j0 + u < k)
                         {
-                            oi[j0 + u] = (uint32_t)kc[u];
+                            oi[j0 + u] = __float_as_uint(pc[u].w);
                             if (od) od[j0 + u] = __uint_as_float((uint32_t)(kc[u] >> 32));
                         }
                 }

@@ -313,3 +313,33 @@ def test_knn_k50_row_staging_changes_speed_only(staging):
     np.testing.assert_array_equal(nn_d.cpu().numpy().view(np.uint32), gi.reshape(-1))
     nn, ptr = scenes.knn_csr(gi)
     np.testing.assert_array_equal(f_d.cpu().numpy(), pgeof.compute_features(xyz, nn, ptr, 50))
+
+
+@pytest.mark.parametrize("k", [26, 50])
+def test_fused_heap_tail_lattice_ties(k):
+    """The fused path at k > 21 keys its budgeted pass by sorted position and
+    sends every query with an exact distance tie to the index-keyed pass: on
+    a lattice (ties everywhere) the rows must still be the oracle's (ties by
+    index) and the features those of compute_features on that CSR."""
+    from point_geometric_features_amd import capi
+
+    g = np.stack(np.meshgrid(*[np.arange(16)] * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    rng = np.random.default_rng(37)
+    xyz = np.ascontiguousarray(g[rng.permutation(len(g))] * np.float32(0.25))
+    x = torch.from_numpy(xyz).cuda()
+    nn_d = torch.empty(len(xyz) * k, dtype=torch.int32, device="cuda")
+    ptr_d = torch.empty(len(xyz) + 1, dtype=torch.int32, device="cuda")
+    f_d = torch.empty((len(xyz), 11), dtype=torch.float32, device="cuda")
+    d_d = torch.empty(len(xyz) * k, dtype=torch.float32, device="cuda")
+    old = capi.get_tuning("heap_tail")
+    try:
+        capi.set_tuning("heap_tail", 2)  # the wave-slot tails of large searches, at this size too
+        capi.knn_features_device(x, k, nn_d, ptr_d, f_d, sqr_dist=d_d)
+        torch.cuda.synchronize()
+    finally:
+        capi.set_tuning("heap_tail", old)
+    oi, od = O.knn(xyz, xyz, k, kdtree=False)
+    np.testing.assert_array_equal(nn_d.cpu().numpy().view(np.uint32).reshape(-1, k), oi)
+    np.testing.assert_array_equal(d_d.cpu().numpy().reshape(-1, k), od)
+    nn, ptr = scenes.knn_csr(oi)
+    np.testing.assert_array_equal(f_d.cpu().numpy(), pgeof.compute_features(xyz, nn, ptr, k))
