@@ -1414,6 +1414,10 @@ int pgeof_radius_multiscale_device(const float* xyz, int64_t n, float search_rad
         a.n_scales = (uint32_t)n_scales;
         a.out = features;
         a.err = d_err;
+        // rows in the search's spatial order (the sorted points of the self
+        // search), kept in the query-cell array a self search does not use
+        HIP_TRY(launch_order_from_sorted(w.pts[0], (uint32_t)n, w.qcell, s));
+        a.order = w.qcell;
         HIP_TRY(launch_csr_features(1, a, s));
     }
     return PGEOF_OK;
