@@ -1665,33 +1665,68 @@ int pgeof_autotune(const char* workload, const float* xyz, int64_t n, uint32_t k
         return pgeof_radius_multiscale_device(xyz, n, radius, k, k_scales, n_scales, (uint32_t*)d_nn,
                                               (uint32_t*)d_ptr, (float*)d_f, d_ws, ws_bytes, s);
     };
+    // one timed run of candidate c (its knobs for the duration of the call)
+    auto time_once = [&](size_t c, float* v) -> int {
+        t_snap = &cand[c];
+        hipError_t e = hipEventRecord(e0, s);
+        int rc = e == hipSuccess ? run() : 0;
+        if (!rc && e == hipSuccess) e = hipEventRecord(e1, s);
+        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+        if (!rc && e == hipSuccess) e = hipEventElapsedTime(v, e0, e1);
+        t_snap = nullptr;
+        if (!rc && e != hipSuccess) rc = hip_fail(e, "autotune timing");
+        return rc;
+    };
+    auto median = [](std::vector<float>& v) -> double {
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
+    };
+    // Stage 1: every candidate, a warm-up and `reps` timed runs (median).
     std::vector<double> ms(cand.size(), 0.0);
-    size_t best = 0;
     for (size_t c = 0; c < cand.size(); ++c)
     {
         t_snap = &cand[c];
         int rc = run();  // warm-up
+        t_snap = nullptr;
         std::vector<float> tt;
         for (int r = 0; r < reps && !rc; ++r)
         {
-            he = hipEventRecord(e0, s);
-            if (he == hipSuccess) rc = run();
-            if (!rc && he == hipSuccess) he = hipEventRecord(e1, s);
-            if (!rc && he == hipSuccess) he = hipEventSynchronize(e1);
             float v = 0.0f;
-            if (!rc && he == hipSuccess) he = hipEventElapsedTime(&v, e0, e1);
-            if (he != hipSuccess) rc = hip_fail(he, "autotune timing");
+            rc = time_once(c, &v);
             tt.push_back(v);
         }
-        t_snap = nullptr;
         if (rc)
         {
             cleanup();
             return rc;
         }
-        std::sort(tt.begin(), tt.end());
-        ms[c] = tt[tt.size() / 2];
-        if (ms[c] < ms[best]) best = c;
+        ms[c] = median(tt);
+    }
+    // Stage 2: the four fastest again, 2 * reps + 1 runs each, taken in
+    // turns so that clock drift affects them alike; the lowest median wins
+    // (close candidates are otherwise picked by noise).
+    std::vector<size_t> order(cand.size());
+    for (size_t c = 0; c < order.size(); ++c) order[c] = c;
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return ms[x] < ms[y]; });
+    const size_t n_fin = std::min<size_t>(4, order.size());
+    std::vector<std::vector<float>> fin(n_fin);
+    for (int r = 0; r < 2 * reps + 1; ++r)
+        for (size_t f = 0; f < n_fin; ++f)
+        {
+            float v = 0.0f;
+            const int rc = time_once(order[f], &v);
+            if (rc)
+            {
+                cleanup();
+                return rc;
+            }
+            fin[f].push_back(v);
+        }
+    size_t best = order[0];
+    for (size_t f = 0; f < n_fin; ++f)
+    {
+        ms[order[f]] = median(fin[f]);
+        if (ms[order[f]] < ms[best]) best = order[f];
     }
     cleanup();
     const int sc = size_class(n);
