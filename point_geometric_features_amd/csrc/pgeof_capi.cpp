@@ -1703,8 +1703,9 @@ int pgeof_autotune(const char* workload, const float* xyz, int64_t n, uint32_t k
         ms[c] = median(tt);
     }
     // Stage 2: the four fastest again, 2 * reps + 1 runs each, taken in
-    // turns so that clock drift affects them alike; the lowest median wins
-    // (close candidates are otherwise picked by noise).
+    // turns so that clock drift affects them alike; their medians replace
+    // the first ones and the lowest time wins (close candidates are
+    // otherwise picked by the noise of a few runs).
     std::vector<size_t> order(cand.size());
     for (size_t c = 0; c < order.size(); ++c) order[c] = c;
     std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return ms[x] < ms[y]; });
@@ -1722,12 +1723,10 @@ int pgeof_autotune(const char* workload, const float* xyz, int64_t n, uint32_t k
             }
             fin[f].push_back(v);
         }
-    size_t best = order[0];
-    for (size_t f = 0; f < n_fin; ++f)
-    {
-        ms[order[f]] = median(fin[f]);
-        if (ms[order[f]] < ms[best]) best = order[f];
-    }
+    for (size_t f = 0; f < n_fin; ++f) ms[order[f]] = median(fin[f]);
+    size_t best = 0;
+    for (size_t c = 1; c < ms.size(); ++c)
+        if (ms[c] < ms[best]) best = c;
     cleanup();
     const int sc = size_class(n);
     {
