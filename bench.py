@@ -14,7 +14,11 @@ CSR feature kernel alone on a precomputed kNN CSR), host_api_lidar10m_k20
 (the Python API on host numpy arrays, PCIe copies included), serving (a
 stream of clouds of 5k..2M points through the serving loop), selected_1m
 (compute_features_selected on 1 M float64 points, the README example's
-radius, cap and features at its point density, host arrays).
+radius, cap and features at its point density, host arrays), ref_knn_1m_k50
+and ref_radius_1m (the reference's own benchmark cases, tests/bench_knn.py:
+1 M points uniform in [0, 200)^3, knn_search k=50 and radius_search r=0.2
+max 30; device-resident, with the host-array call, the oracle's KD-tree and
+scipy's KDTree timed beside it and every query checked bit for bit).
 
 The default line also carries the serving stream's results (key "serving":
 2,000 requests, p50/p99 latency, open loop vs one request at a time) and,
@@ -172,6 +176,45 @@ def _cpu_scipy(xyz: np.ndarray, k: int, sample: int):
     return {"value": len(xyz) / total / 1e6, "unit": "Mpoints/s", "cores": cores,
             "sample": f"cKDTree build over all points ({t1 - t0:.2f} s) + k={k} query of {len(q)} points "
                       f"({t2 - t1:.2f} s), scaled; kNN only"}
+
+
+def _cpu_ref_bench(xyz: np.ndarray, k: int, r, idx, d2):
+    """The reference benchmark's CPU sides on every query (tests/bench_knn.py):
+    the oracle's KD-tree (C restatement of nanoflann's search, all host
+    cores) as the baseline, scipy's KDTree (the benchmark's own comparator)
+    beside it, and the GPU's indices and squared distances against the
+    oracle's over all queries, bit for bit."""
+    from scipy.spatial import KDTree
+
+    from oracle import oracle as O
+
+    cores, model = _host_cpu()
+    t0 = time.perf_counter()
+    if r is None:
+        ri, rd = O.knn(xyz, xyz, k, kdtree=True, n_threads=cores)
+    else:
+        ri, rd, _ = O.radius(xyz, xyz, r, k, kdtree=True, n_threads=cores)
+    t1 = time.perf_counter()
+    tree = KDTree(xyz)
+    if r is None:
+        tree.query(xyz, k=k, workers=cores)
+    else:
+        tree.query(xyz, k=k, distance_upper_bound=r, workers=cores)
+    t2 = time.perf_counter()
+    gi = idx.cpu().numpy()
+    gd = d2.cpu().numpy()
+    if r is None:
+        gi = gi.view(np.uint32)
+    parity = {"queries": int(len(xyz)), "indices_equal": bool(np.array_equal(gi, ri)),
+              "sqr_dist_equal": bool(np.array_equal(gd, rd))}
+    what = f"knn k={k}" if r is None else f"radius r={r} max {k}"
+    cpu = {"value": len(xyz) / (t1 - t0) / 1e6, "unit": "Mpoints/s", "cores": cores, "cpu": model,
+           "kind": "port",
+           "sample": f"the oracle's KD-tree (C restatement), build + {what} of all {len(xyz)} points "
+                     f"({t1 - t0:.2f} s, {cores} threads); CPU restatement, not the reference binary",
+           "scipy_kdtree": {"value": round(len(xyz) / (t2 - t1) / 1e6, 3), "unit": "Mpoints/s",
+                            "seconds": round(t2 - t1, 2), "workers": cores}}
+    return cpu, parity
 
 
 def _copy_gbs(torch, dev) -> float:
@@ -640,6 +683,42 @@ def main():
         kernel = "features_selected_f64"
         bytes_per_step = (24 + 8 * len(fids)) * n  # xyz f64 in, selected features f64 out
         units = n
+    elif wl in ("ref_knn_1m_k50", "ref_radius_1m"):
+        # the reference's own benchmark cases (tests/bench_knn.py:8-54): 1 M
+        # points uniform in [0, 200)^3, every point a query; knn_search k = 50,
+        # radius_search r = 0.2 with at most 30 neighbours (indices and squared
+        # distances out, as the reference's functions return them)
+        n = 1_000_000
+        xyz = np.random.default_rng(6).uniform(0.0, 200.0, size=(n, 3)).astype(np.float32)
+        x = torch.from_numpy(xyz).to(dev)
+        if wl == "ref_knn_1m_k50":
+            k, r = 50, None
+            cfg.update({"points": n, "k": k, "path": "knn_search, device-resident (pgeof_knn_search_device)"})
+            tuned = None if args.no_autotune else capi.autotune("knn_features", x, k)
+            ws = torch.empty(capi.knn_workspace_size(n, n, k), dtype=torch.uint8, device=dev)
+        else:
+            k, r = 30, 0.2
+            cfg.update({"points": n, "radius": r, "max_knn": k,
+                        "path": "radius_search, device-resident (pgeof_radius_search_device)"})
+            ws = torch.empty(capi.radius_workspace_size(n, n, k), dtype=torch.uint8, device=dev)
+        idx = torch.empty((n, k), dtype=torch.int32, device=dev)
+        d2 = torch.empty((n, k), dtype=torch.float32, device=dev)
+
+        def step():
+            if r is None:
+                capi.knn_search_device(x, x, k, idx, d2, workspace=ws)
+            else:
+                capi.radius_search_device(x, x, r, k, idx, d2, workspace=ws)
+
+        def cpu_leg():
+            return _cpu_ref_bench(xyz, k, r, idx, d2)
+
+        def host_api():
+            return pgeof.knn_search(xyz, xyz, k) if r is None else pgeof.radius_search(xyz, xyz, r, k)
+
+        kernel = ("knn_search_dense", "knn_search") if r is None else "radius_search"
+        bytes_per_step = (12 + 8 * k) * n  # xyz in, k indices + k squared distances out per point
+        units = n
     else:
         raise SystemExit(f"unknown workload {wl}")
     cfg["inputs_sha256_16"] = _sha(xyz)
@@ -702,9 +781,25 @@ def main():
     if not args.no_cpu_baseline and cpu_leg is not None:
         cpu, ferr = cpu_leg()
         _progress(f"{wl}: CPU baseline done")
-        extra["feature_err_vs_fp64_oracle"] = ferr
+        if wl in ("ref_knn_1m_k50", "ref_radius_1m"):
+            extra["parity_vs_oracle"] = ferr
+        else:
+            extra["feature_err_vs_fp64_oracle"] = ferr
         if wl in ("lidar10m_k20", "lidar10m_k50"):
             extra["cpu_scipy_knn"] = _cpu_scipy(xyz, k, args.cpu_sample)
+    if wl in ("ref_knn_1m_k50", "ref_radius_1m"):
+        # the reference's benchmark calls the Python functions on host arrays:
+        # the same call here, PCIe copies of the 400 / 240 MB of results included
+        host_api()
+        th = []
+        for _ in range(3):
+            ts = time.perf_counter()
+            host_api()
+            th.append(time.perf_counter() - ts)
+        extra["host_api"] = {"ms": round(float(np.median(th)) * 1e3, 2),
+                             "mpts_s": round(units / float(np.median(th)) / 1e6, 1),
+                             "call": "pgeof.knn_search(xyz, xyz, 50)" if wl == "ref_knn_1m_k50"
+                             else "pgeof.radius_search(xyz, xyz, 0.2, 30)"}
     if roofline is not None:
         roofline["copy_gbs"] = round(_copy_gbs(torch, dev), 1)
 

@@ -693,6 +693,7 @@ struct Heap
     }
 
     // pad with the sentinel, restore the heap property, sort ascending
+    template <bool SHORT = false>
     __device__ __forceinline__ void finish(uint64_t sentinel)
     {
         if (count < k)
@@ -729,7 +730,8 @@ struct Heap4
     uint64_t* keys;    // this lane's pair 0; pair p at keys + p * pstride
     uint32_t pstride;  // entries between consecutive pairs of one lane (2 * block size)
     uint32_t k;
-    uint32_t count;    // keys inserted, up to k
+    uint32_t count;    // keys inserted, up to k (after finish(): entries held, key_at past them = sent)
+    uint64_t sent;     // the sentinel
     uint64_t* gkeys;   // this lane's first global pair; pair p at gkeys + p * gpstride
     size_t gpstride;   // entries between consecutive global pairs of one lane
     uint64_t rkey;     // node 0
@@ -779,7 +781,9 @@ struct Heap4
         n3 = (kk & m3) | (n3 & ~m3);
         n4 = (kk & m4) | (n4 & ~m4);
     }
-    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return i <= 4 ? reg(i) : load(i); }
+    __device__ __forceinline__ uint64_t node(uint32_t i) const { return i <= 4 ? reg(i) : load(i); }
+    // entry i of the sorted heap (after finish)
+    __device__ __forceinline__ uint64_t key_at(uint32_t i) const { return i < count ? node(i) : sent; }
     __device__ __forceinline__ void put(uint32_t i, uint64_t kk)
     {
         if (i <= 4)
@@ -866,6 +870,7 @@ struct Heap4
     __device__ __forceinline__ void init(uint64_t sentinel)
     {
         count = 0;
+        sent = sentinel;
         rkey = n1 = n2 = n3 = n4 = sentinel;
     }
 
@@ -880,8 +885,36 @@ struct Heap4
     }
 
     // sort ascending (entries never filled keep the sentinel)
+    // SHORT: sort rows of few keys in registers (radius searches)
+    static constexpr uint32_t kShortFinish = 8;
+    template <bool SHORT = false>
     __device__ __forceinline__ void finish(uint64_t sentinel)
     {
+        if (SHORT && count < k && count <= kShortFinish)
+        {
+            // Few keys (a radius search's sparse rows): they sit in the fill's
+            // nodes k-count..k-1; sort them in registers (a 19-comparator
+            // network for 8) into nodes 0..count-1, and key_at() answers the
+            // sentinel past them, instead of a heap sort of all k nodes that
+            // sifts sentinels through LDS and global memory.
+            uint64_t v[kShortFinish];
+#pragma unroll
+            for (uint32_t u = 0; u < kShortFinish; ++u) v[u] = u < count ? node(k - count + u) : ~0ull;
+            constexpr int kNet[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6},
+                                         {3, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}, {2, 4}, {3, 5},
+                                         {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+            for (int c = 0; c < 19; ++c)
+            {
+                const uint64_t x = v[kNet[c][0]], y = v[kNet[c][1]];
+                v[kNet[c][0]] = x < y ? x : y;
+                v[kNet[c][1]] = x < y ? y : x;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kShortFinish; ++u)
+                if (u < count) put(u, v[u]);
+            return;
+        }
         // memory nodes 5..k-1-count were never reached by the fill
         for (uint32_t i = 5; i + count < k; ++i) store(i, sentinel);
         uint32_t end = k - 1;
@@ -897,6 +930,7 @@ struct Heap4
             set_reg(end, rkey);
             sift_from(0, last, end);
         }
+        count = k;
     }
 };
 
@@ -913,6 +947,49 @@ __device__ __forceinline__ uint32_t hw_wave_slot()
     const uint32_t wave = hw & 15u, simd = (hw >> 4) & 3u, cu = (hw >> 8) & 15u, sh = (hw >> 12) & 1u,
                    se = (hw >> 13) & 3u;
     return (((((xcc & (kHwMaxXcc - 1)) * 4u + se) * 2u + sh) * 16u + cu) * 4u + simd) * 16u + wave;
+}
+
+// A query's row of n 32-bit entries in oi (and, when od is set, the row at
+// the same offset of od), 16 bytes per store: the entries up to oi's next
+// 16-byte boundary one by one, then four per store, then the rest.  One
+// lane writes one row and the rows of a wave's lanes are far apart, so a
+// store of the wave touches as many cache lines as it has lanes: writing
+// four entries per store cuts those requests fourfold.  (od not aligned
+// like oi: every entry on its own.)  emit(j, i, d) gives entry j.
+template <class Emit>
+__device__ __forceinline__ void store_rows(uint32_t* __restrict__ oi, float* __restrict__ od, uint32_t n, Emit emit)
+{
+    const uintptr_t ai = reinterpret_cast<uintptr_t>(oi);
+    const bool same = od == nullptr || ((ai ^ reinterpret_cast<uintptr_t>(od)) & 15u) == 0;
+    const uint32_t head = same ? min((uint32_t)((16u - (ai & 15u)) & 15u) >> 2, n) : n;
+    uint32_t j = 0;
+    for (; j < head; ++j)
+    {
+        uint32_t i;
+        float d;
+        emit(j, i, d);
+        oi[j] = i;
+        if (od) od[j] = d;
+    }
+    for (; j + 4 <= n; j += 4)
+    {
+        uint32_t i0, i1, i2, i3;
+        float d0, d1, d2, d3;
+        emit(j, i0, d0);
+        emit(j + 1, i1, d1);
+        emit(j + 2, i2, d2);
+        emit(j + 3, i3, d3);
+        *reinterpret_cast<uint4*>(oi + j) = make_uint4(i0, i1, i2, i3);
+        if (od) *reinterpret_cast<float4*>(od + j) = make_float4(d0, d1, d2, d3);
+    }
+    for (; j < n; ++j)
+    {
+        uint32_t i;
+        float d;
+        emit(j, i, d);
+        oi[j] = i;
+        if (od) od[j] = d;
+    }
 }
 
 // distance from v to the slab [lo + c h, lo + (c+1) h], less a safety margin
@@ -1567,53 +1644,44 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
             return;
         }
     }
-    if constexpr (!POSKEY) heap.finish(sentinel);
+    // keys held (radius searches: the hits, at most k; they come first in
+    // the sorted heap, the sentinels after them)
+    const uint32_t hits = heap.count;
+    if constexpr (!POSKEY) heap.template finish<RADIUS>(sentinel);
     const uint32_t k = a.k;
 
     if (MODE == kModeKnn)
     {
         uint32_t* oi = reinterpret_cast<uint32_t*>(a.out_idx) + (size_t)qorig * k;
         float* od = a.out_d2 ? a.out_d2 + (size_t)qorig * k : nullptr;
-        for (uint32_t j = 0; j < k; ++j)
-        {
+        store_rows(oi, od, k, [&](uint32_t j, uint32_t& i, float& d) {
             const uint64_t key = heap.key_at(j);
-            oi[j] = (uint32_t)key;
-            if (od) od[j] = key == ~0ull ? INFINITY : __uint_as_float((uint32_t)(key >> 32));
-        }
+            i = (uint32_t)key;
+            d = key == ~0ull ? INFINITY : __uint_as_float((uint32_t)(key >> 32));
+        });
     }
     else if (MODE == kModeRadius || MODE == kModeRadiusCsr || MODE == kModeRadiusMulti)
     {
-        uint32_t cnt = 0;
+        const uint32_t cnt = hits;
         if (MODE == kModeRadius)
         {
-            int32_t* oi = reinterpret_cast<int32_t*>(a.out_idx) + (size_t)qorig * k;
+            uint32_t* oi = reinterpret_cast<uint32_t*>(a.out_idx) + (size_t)qorig * k;
             float* od = a.out_d2 + (size_t)qorig * k;
-            for (uint32_t j = 0; j < k; ++j)
-            {
-                const uint64_t key = heap.key_at(j);
-                if (key != sentinel)
-                {
-                    oi[j] = (int32_t)(uint32_t)key;
-                    od[j] = __uint_as_float((uint32_t)(key >> 32));
-                    ++cnt;
-                }
-                else
-                {
-                    oi[j] = -1;
-                    od[j] = 0.0f;
-                }
-            }
-            // nanoflann's RKNNResultSet::init stores r^2 in the last slot
-            if (cnt < k) od[k - 1] = a.r2;
+            // -1 / 0 past the hits, and nanoflann's RKNNResultSet::init
+            // leaves r^2 in the last slot when it is not a hit
+            store_rows(oi, od, k, [&](uint32_t j, uint32_t& i, float& d) {
+                const uint64_t key = j < cnt ? heap.key_at(j) : sentinel;
+                i = j < cnt ? (uint32_t)key : 0xFFFFFFFFu;
+                d = j < cnt ? __uint_as_float((uint32_t)(key >> 32)) : (j == k - 1 ? a.r2 : 0.0f);
+            });
         }
         else
         {
             uint32_t* oi = reinterpret_cast<uint32_t*>(a.out_idx) + (size_t)qorig * k;
-            for (uint32_t j = 0; j < k; ++j)
-            {
-                const uint64_t key = heap.key_at(j);
-                if (key != sentinel) oi[cnt++] = (uint32_t)key;
-            }
+            store_rows(oi, nullptr, cnt, [&](uint32_t j, uint32_t& i, float& d) {
+                i = (uint32_t)heap.key_at(j);
+                d = 0.0f;
+            });
         }
         if (a.out_count) a.out_count[qorig] = cnt;
         if (MODE == kModeRadiusMulti)
@@ -1674,7 +1742,10 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
             // four issued before the current ones are accumulated.  Rows of
             // the index (and distance) output are written four entries per
             // store when k is a multiple of 4 and the arrays are 16-byte
-            // aligned (every row is then aligned too).
+            // aligned (every row is then aligned too), two per store when k
+            // is even and the arrays 8-byte aligned (the rows of a wave's
+            // lanes are far apart, so a store reaches as many cache lines as
+            // the wave has lanes: fewer, wider stores make fewer requests).
             Moments m;
             mom_zero(m);
             const float3* __restrict__ xyz3 = reinterpret_cast<const float3*>(a.data);
@@ -1690,8 +1761,9 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                 }
             };
             constexpr uint32_t kChunk = 4;
-            const bool vec_out =
-                (k & 3u) == 0 && ((reinterpret_cast<uintptr_t>(a.out_idx) | reinterpret_cast<uintptr_t>(a.out_d2)) & 15u) == 0;
+            const uintptr_t aln = reinterpret_cast<uintptr_t>(a.out_idx) | reinterpret_cast<uintptr_t>(a.out_d2);
+            const bool vec_out = (k & 3u) == 0 && (aln & 15u) == 0;
+            const bool vec2_out = !vec_out && (k & 1u) == 0 && (aln & 7u) == 0;
             uint64_t kn[kChunk];
             float4 pn[kChunk];
 #pragma unroll
@@ -1730,6 +1802,21 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                         *reinterpret_cast<float4*>(od + j0) = make_float4(
                             __uint_as_float((uint32_t)(kc[0] >> 32)), __uint_as_float((uint32_t)(kc[1] >> 32)),
                             __uint_as_float((uint32_t)(kc[2] >> 32)), __uint_as_float((uint32_t)(kc[3] >> 32)));
+                }
+                else if (vec2_out)
+                {
+                    // k even: entries j0, j0 + 1 exist, and j0 + 2, j0 + 3 together
+#pragma unroll
+                    for (uint32_t u = 0; u < kChunk; u += 2)
+                        if (j0 + u < k)
+                        {
+                            *reinterpret_cast<uint2*>(oi + j0 + u) =
+                                make_uint2(__float_as_uint(pc[u].w), __float_as_uint(pc[u + 1].w));
+                            if (od)
+                                *reinterpret_cast<float2*>(od + j0 + u) =
+                                    make_float2(__uint_as_float((uint32_t)(kc[u] >> 32)),
+                                                __uint_as_float((uint32_t)(kc[u + 1] >> 32)));
+                        }
                 }
                 else
                 {

@@ -83,3 +83,25 @@ def test_cross_cloud_knn_radius_1m():
     oi, od, _ = O.radius(data, q, 0.4, 32, kdtree=True)
     np.testing.assert_array_equal(ri, oi)
     np.testing.assert_array_equal(rd, od)
+
+
+def test_reference_benchmark_cases_1m():
+    """The reference's own benchmark cases (tests/bench_knn.py:8-54): 1 M
+    points uniform in [0, 200)^3, every point a query, knn_search k = 50 and
+    radius_search r = 0.2 with at most 30 neighbours through the pgeof API;
+    every row bit-exact against the oracle's KD-tree (at this density almost
+    every radius row holds only its own point: -1 / 0 padding on 99.6 % of
+    the rows)."""
+    xyz = np.random.default_rng(6).uniform(0.0, 200.0, size=(1_000_000, 3)).astype(np.float32)
+    print("fullsize: reference benchmark cases, 1M uniform [0, 200)^3", flush=True)
+    import pgeof
+
+    gi, gd = pgeof.knn_search(xyz, xyz, 50)
+    oi, od = O.knn(xyz, xyz, 50, kdtree=True)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    ri, rd = pgeof.radius_search(xyz, xyz, 0.2, 30)
+    oi, od, cnt = O.radius(xyz, xyz, 0.2, 30, kdtree=True)
+    np.testing.assert_array_equal(ri, oi)
+    np.testing.assert_array_equal(rd, od)
+    assert (cnt >= 1).all() and 0 < (cnt > 1).mean() < 0.01
