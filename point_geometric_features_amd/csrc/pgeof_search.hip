@@ -581,15 +581,26 @@ __global__ __launch_bounds__(256) void k_compact_rows(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ row_ptr,
                                                       uint32_t* __restrict__ nn)
 {
-    // one wave per row, lanes along the row: only the filled entries are
-    // read, both sides coalesced, and no 64-bit division per entry
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, n_waves = gridDim.x * 4u;
-    for (uint32_t r = wave; r < n_rows; r += n_waves)
+    // 16 lanes per row (four rows per wave), lanes along the row: only the
+    // filled entries are read, both sides coalesced, no 64-bit division per
+    // entry; a lane issues up to four loads before its stores, and the four
+    // rows of a wave wait for their counts and offsets together
+    const uint32_t sub = threadIdx.x & 15u;
+    const uint32_t grp = (blockIdx.x * 256u + threadIdx.x) >> 4, n_grp = gridDim.x * 16u;
+    for (uint32_t r = grp; r < n_rows; r += n_grp)
     {
         const uint32_t c = min(counts[r], cap), base = row_ptr[r];
         const uint32_t* __restrict__ src = dense + (size_t)r * cap;
-        for (uint32_t j = lane; j < c; j += 64u) nn[base + j] = src[j];
+        uint32_t j = sub;
+        for (; j + 48u < c; j += 64u)
+        {
+            const uint32_t v0 = src[j], v1 = src[j + 16u], v2 = src[j + 32u], v3 = src[j + 48u];
+            nn[base + j] = v0;
+            nn[base + j + 16u] = v1;
+            nn[base + j + 32u] = v2;
+            nn[base + j + 48u] = v3;
+        }
+        for (; j < c; j += 16u) nn[base + j] = src[j];
     }
 }
 
@@ -2527,7 +2538,7 @@ hipError_t launch_compact_rows(const uint32_t* dense, uint32_t n_rows, uint32_t 
 {
     if ((uint64_t)n_rows * cap == 0) return hipSuccess;
     ProfScope ps("radius_compact", s);
-    k_compact_rows<<<blocks_for((uint64_t)n_rows * 64, 256, 65536), 256, 0, s>>>(dense, n_rows, cap, counts,
+    k_compact_rows<<<blocks_for((uint64_t)n_rows * 16, 256, 65536), 256, 0, s>>>(dense, n_rows, cap, counts,
                                                                                 row_ptr, nn);
     return hipGetLastError();
 }
