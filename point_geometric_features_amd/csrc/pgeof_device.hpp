@@ -375,9 +375,11 @@ __host__ __device__ __forceinline__ float sqdist32(float qx, float qy, float qz,
     return d;
 }
 
-// The 11 features of a row in five stores instead of eleven: the scalars up
-// to the next 16-byte boundary, two 16-byte stores, the scalar rest.  (Rows
-// are 44 bytes, so a row starts 0, 4, 8 or 12 bytes past a boundary.)
+// The 11 features of a row in three or four stores instead of eleven: two
+// 16-byte stores at the row's 16-byte boundaries, the entries before and
+// after them in one 4-, 8- or 12-byte store each.  (Rows are 44 bytes, so a
+// row starts 0, 4, 8 or 12 bytes past a boundary.  The rows of a wave's
+// lanes are far apart: every store reaches one cache line per lane.)
 __device__ __forceinline__ void store_row11(float* __restrict__ o, const float f[11])
 {
     switch ((4u - ((uint32_t)(reinterpret_cast<uintptr_t>(o) >> 2) & 3u)) & 3u)
@@ -385,22 +387,22 @@ __device__ __forceinline__ void store_row11(float* __restrict__ o, const float f
         case 0:
             *reinterpret_cast<float4*>(o) = make_float4(f[0], f[1], f[2], f[3]);
             *reinterpret_cast<float4*>(o + 4) = make_float4(f[4], f[5], f[6], f[7]);
-            o[8] = f[8], o[9] = f[9], o[10] = f[10];
+            *reinterpret_cast<float3*>(o + 8) = make_float3(f[8], f[9], f[10]);
             break;
         case 1:
             o[0] = f[0];
             *reinterpret_cast<float4*>(o + 1) = make_float4(f[1], f[2], f[3], f[4]);
             *reinterpret_cast<float4*>(o + 5) = make_float4(f[5], f[6], f[7], f[8]);
-            o[9] = f[9], o[10] = f[10];
+            *reinterpret_cast<float2*>(o + 9) = make_float2(f[9], f[10]);
             break;
         case 2:
-            o[0] = f[0], o[1] = f[1];
+            *reinterpret_cast<float2*>(o) = make_float2(f[0], f[1]);
             *reinterpret_cast<float4*>(o + 2) = make_float4(f[2], f[3], f[4], f[5]);
             *reinterpret_cast<float4*>(o + 6) = make_float4(f[6], f[7], f[8], f[9]);
             o[10] = f[10];
             break;
         default:
-            o[0] = f[0], o[1] = f[1], o[2] = f[2];
+            *reinterpret_cast<float3*>(o) = make_float3(f[0], f[1], f[2]);
             *reinterpret_cast<float4*>(o + 3) = make_float4(f[3], f[4], f[5], f[6]);
             *reinterpret_cast<float4*>(o + 7) = make_float4(f[7], f[8], f[9], f[10]);
             break;
