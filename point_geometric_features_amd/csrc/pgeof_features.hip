@@ -245,21 +245,35 @@ __global__ __launch_bounds__(256, PGEOF_CSR_WAVES_PER_SIMD) void k_csr(CsrArgs a
 // gets most bits in x and y.  Equal bits per axis would leave the ground of
 // such a scene in one z slice, with hundreds of rows per bucket and the
 // histogram's atomics serialised on them.
-constexpr int kOrderBits = 21;  // 2^21 buckets
+//
+// The rows are sorted by key in two counting passes, most significant bits
+// first, with no global atomics: rows are ranked within blocks of
+// kOrderRows by the key's high kOrderBits - kLoBits bits (LDS atomics), the
+// blocks' counts per high bucket scanned bucket-major give every row its
+// place among the rows of its high bucket, and one workgroup per high
+// bucket then sorts those rows by the low kLoBits bits in LDS.  (A single
+// pass over 2^21 buckets took one returning global atomic per row: 0.45 ms
+// of the order's 0.76 ms per 10 M rows.)
+constexpr int kOrderBits = 20;  // 2^20 buckets
+constexpr int kLoBits = 10;
+constexpr uint32_t kHiBins = 1u << (kOrderBits - kLoBits), kLoBins = 1u << kLoBits;
+constexpr uint32_t kOrderBlock = 1024;  // threads of the two LDS passes (== kLoBins)
+constexpr uint32_t kOrderPer = 4;       // rows per thread of the first pass
+constexpr uint32_t kOrderRows = kOrderBlock * kOrderPer;
+constexpr int kRankBits = 12;  // rank of a row within its block and high bucket (< kOrderRows)
+static_assert(kLoBins == kOrderBlock && kOrderRows == (1u << kRankBits) && kOrderBits + kRankBits <= 32,
+              "row-order packing");
 
 __device__ __forceinline__ int longest_axis(const float cell[3])
 {
     return cell[0] >= cell[1] ? (cell[0] >= cell[2] ? 0 : 2) : (cell[1] >= cell[2] ? 1 : 2);
 }
 
-__global__ __launch_bounds__(256) void k_row_keys(const float* __restrict__ xyz, uint32_t n_xyz,
-                                                  const uint32_t* __restrict__ nn, uint64_t n_nn,
-                                                  const uint32_t* __restrict__ nn_ptr, uint32_t n_points,
-                                                  const float* __restrict__ bbox, uint32_t* __restrict__ key,
-                                                  uint32_t* __restrict__ count, uint32_t* __restrict__ rank)
+__device__ __forceinline__ uint32_t row_key(const float* __restrict__ xyz, uint32_t n_xyz,
+                                            const uint32_t* __restrict__ nn, uint64_t n_nn,
+                                            const uint32_t* __restrict__ nn_ptr, const float* __restrict__ bbox,
+                                            uint32_t i)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n_points) return;
     uint32_t j = i;
     if (i >= n_xyz)
     {
@@ -301,57 +315,141 @@ __global__ __launch_bounds__(256) void k_row_keys(const float* __restrict__ xyz,
             cell[ax] *= 0.5f;
         }
     }
-    key[i] = k;
-    rank[i] = atomicAdd(&count[k], 1u);  // histogram of the keys, rank within the bucket
+    return k;
 }
 
-__global__ __launch_bounds__(256) void k_key_scatter(const uint32_t* __restrict__ key,
-                                                     const uint32_t* __restrict__ rank, uint32_t n,
-                                                     const uint32_t* __restrict__ start, uint32_t* __restrict__ order)
+// First pass: every row's key and rank among the rows of its block with the
+// same high bucket, packed as (key << kRankBits) | rank; the block's count
+// per high bucket at bcount[bucket * n_blocks + block] (one more entry, the
+// last, is 0: its exclusive scan is the number of rows).
+__global__ __launch_bounds__(kOrderBlock) void k_row_keys(const float* __restrict__ xyz, uint32_t n_xyz,
+                                                          const uint32_t* __restrict__ nn, uint64_t n_nn,
+                                                          const uint32_t* __restrict__ nn_ptr, uint32_t n_points,
+                                                          const float* __restrict__ bbox, uint32_t* __restrict__ kr,
+                                                          uint32_t* __restrict__ bcount, uint32_t n_blocks)
+{
+    __shared__ uint32_t hist[kHiBins];
+    for (uint32_t b = threadIdx.x; b < kHiBins; b += kOrderBlock) hist[b] = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t t = 0; t < kOrderPer; ++t)
+    {
+        const uint32_t i = blockIdx.x * kOrderRows + t * kOrderBlock + threadIdx.x;
+        if (i < n_points)
+        {
+            const uint32_t k = row_key(xyz, n_xyz, nn, n_nn, nn_ptr, bbox, i);
+            const uint32_t r = atomicAdd(&hist[k >> kLoBits], 1u);
+            kr[i] = (k << kRankBits) | r;
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kHiBins; b += kOrderBlock) bcount[(size_t)b * n_blocks + blockIdx.x] = hist[b];
+    if (blockIdx.x == 0 && threadIdx.x == 0) bcount[(size_t)kHiBins * n_blocks] = 0;
+}
+
+// Second pass: each row to its place among the rows of its high bucket
+// (bucket-major offsets), as (row, low bits of its key).
+__global__ __launch_bounds__(256) void k_row_hi_scatter(const uint32_t* __restrict__ kr, uint32_t n,
+                                                        const uint32_t* __restrict__ boff, uint32_t n_blocks,
+                                                        uint2* __restrict__ tmp)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i < n) order[start[key[i]] + rank[i]] = i;
+    if (i >= n) return;
+    const uint32_t v = kr[i], k = v >> kRankBits, r = v & ((1u << kRankBits) - 1u);
+    const uint32_t pos = boff[(size_t)(k >> kLoBits) * n_blocks + i / kOrderRows] + r;
+    tmp[pos] = make_uint2(i, k & (kLoBins - 1u));
 }
+
+// Third pass: one workgroup per high bucket counts its rows' low bits in
+// LDS, scans the counts and places every row (the rank within a low bucket
+// is the order of the LDS atomics: any order within a bucket will do).
+__global__ __launch_bounds__(kOrderBlock) void k_row_lo_sort(const uint2* __restrict__ tmp,
+                                                             const uint32_t* __restrict__ boff, uint32_t n_blocks,
+                                                             uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t h[kLoBins];
+    __shared__ uint32_t wsum[kOrderBlock / 64];
+    const uint32_t s = boff[(size_t)blockIdx.x * n_blocks], e = boff[(size_t)(blockIdx.x + 1) * n_blocks];
+    if (s >= e) return;  // (uniform over the workgroup)
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    h[tid] = 0;
+    __syncthreads();
+    for (uint32_t x = s + tid; x < e; x += kOrderBlock) atomicAdd(&h[tmp[x].y], 1u);
+    __syncthreads();
+    // exclusive scan of the kLoBins counts, one per thread
+    const uint32_t v = h[tid];
+    uint32_t incl = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1)
+    {
+        const uint32_t t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (w == 0)
+    {
+        const uint32_t ws = lane < kOrderBlock / 64 ? wsum[lane] : 0u;
+        uint32_t wi = ws;
+#pragma unroll
+        for (uint32_t d = 1; d < kOrderBlock / 64; d <<= 1)
+        {
+            const uint32_t t = __shfl_up(wi, d);
+            if (lane >= d) wi += t;
+        }
+        if (lane < kOrderBlock / 64) wsum[lane] = wi - ws;
+    }
+    __syncthreads();
+    h[tid] = incl - v + wsum[w];
+    __syncthreads();
+    for (uint32_t x = s + tid; x < e; x += kOrderBlock)
+    {
+        const uint2 r = tmp[x];
+        order[s + atomicAdd(&h[r.y], 1u)] = r.x;
+    }
+}
+
+uint32_t order_blocks(uint32_t n_points) { return (n_points + kOrderRows - 1) / kOrderRows; }
 
 }  // namespace
 
 size_t csr_order_bytes(uint32_t n_points)
 {
-    const uint64_t buckets = 1ull << kOrderBits;
+    const uint64_t counts = (uint64_t)kHiBins * order_blocks(n_points) + 1;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    return al(1024 * 6 * 4) + al(16 * 4) + 3 * al((size_t)n_points * 4) + al((buckets + 1) * 4) +
-           al(scan_partials_needed(buckets + 1) * 4);
+    return al(1024 * 6 * 4) + al(16 * 4) + al((size_t)n_points * 4) + al((size_t)n_points * 8) + al(counts * 4) +
+           al(scan_partials_needed(counts) * 4);
 }
 
 hipError_t launch_csr_order(const CsrArgs& a, void* ws, uint32_t* order_out_ptr_holder[1], hipStream_t s)
 {
     const uint32_t n = a.n_points;
-    const uint64_t buckets = 1ull << kOrderBits;
+    const uint32_t nb = order_blocks(n);
+    const uint64_t counts = (uint64_t)kHiBins * nb + 1;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     char* p = (char*)ws;
     float* bb_part = (float*)p;
     p += al(1024 * 6 * 4);
     float* bb = (float*)p;
     p += al(16 * 4);
-    uint32_t* key = (uint32_t*)p;
+    // the packed keys, then (once the second pass has read them) the order
+    uint32_t* kr = (uint32_t*)p;
+    uint32_t* order = kr;
     p += al((size_t)n * 4);
-    uint32_t* rank = (uint32_t*)p;
-    p += al((size_t)n * 4);
-    uint32_t* order = (uint32_t*)p;
-    p += al((size_t)n * 4);
-    uint32_t* count = (uint32_t*)p;
-    p += al((buckets + 1) * 4);
+    uint2* tmp = (uint2*)p;
+    p += al((size_t)n * 8);
+    uint32_t* bcount = (uint32_t*)p;
+    p += al(counts * 4);
     uint32_t* part = (uint32_t*)p;
     ProfScope ps("csr_row_order", s);
     const uint32_t nbb = std::max<uint32_t>(1, std::min<uint32_t>(1024, (a.n_xyz + 255) / 256));
     hipError_t e = launch_bbox(a.xyz, a.n_xyz, bb_part, nbb, bb, s);
     if (e != hipSuccess) return e;
-    e = launch_fill_u32(count, buckets + 1, 0u, s);
+    k_row_keys<<<nb, kOrderBlock, 0, s>>>(a.xyz, a.n_xyz, a.nn, a.n_nn, a.nn_ptr, n, bb, kr, bcount, nb);
+    e = launch_scan_exclusive(bcount, bcount, counts, part, s);
     if (e != hipSuccess) return e;
-    k_row_keys<<<(n + 255) / 256, 256, 0, s>>>(a.xyz, a.n_xyz, a.nn, a.n_nn, a.nn_ptr, n, bb, key, count, rank);
-    e = launch_scan_exclusive(count, count, buckets + 1, part, s);
-    if (e != hipSuccess) return e;
-    k_key_scatter<<<(n + 255) / 256, 256, 0, s>>>(key, rank, n, count, order);
+    k_row_hi_scatter<<<(n + 255) / 256, 256, 0, s>>>(kr, n, bcount, nb, tmp);
+    k_row_lo_sort<<<kHiBins, kOrderBlock, 0, s>>>(tmp, bcount, nb, order);
     order_out_ptr_holder[0] = order;
     return hipGetLastError();
 }

@@ -20,7 +20,8 @@ namespace
 {
 
 // 60 MiB: the spatial row order of PGEOF_NOALLOC_MAX_ROWS rows
-// (3 u32 per row + 2^21 + 1 bucket counts + scan partials: 56.1 MiB)
+// (12 bytes per row + 2^10 bucket counts per 4096 rows + scan partials:
+// 52.0 MiB)
 constexpr size_t kScratchBytes = size_t(60) << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 

@@ -226,3 +226,25 @@ def test_csr_spatial_row_order_changes_nothing():
         np.testing.assert_array_equal(a, b)
     ref = O.compute_features(xyz, nn, ptr)
     assert_features_close(res[1][0], ref, O.eigvals_csr_np(xyz, nn, ptr), "spatial row order")
+
+
+def test_row_order_one_bucket():
+    """Every row in one bucket of the spatial row order (all points at one
+    place, plus a far cluster so that the box is not empty): the last pass
+    places 300,000 rows of one bucket; the results equal index order's."""
+    from point_geometric_features_amd import capi
+
+    rng = np.random.default_rng(35)
+    xyz = np.zeros((300_000, 3), np.float32)
+    xyz[:1000] = rng.random((1000, 3), dtype=np.float32) + 100.0
+    nn = ((np.arange(len(xyz))[:, None] + np.arange(8)[None, :]) % len(xyz)).astype(np.uint32).ravel()
+    ptr = (np.arange(len(xyz) + 1) * 8).astype(np.uint32)
+    old = capi.get_tuning("csr_order")
+    try:
+        res = {}
+        for order in (1, 0):
+            capi.set_tuning("csr_order", order)
+            res[order] = pgeof.compute_features(xyz, nn, ptr)
+    finally:
+        capi.set_tuning("csr_order", old)
+    np.testing.assert_array_equal(res[1], res[0])
