@@ -258,3 +258,42 @@ def test_optimal_k100_precision_rows():
         val[sel] = O.eigvals_csr_np(xyz, sub_nn, sub_ptr)
     assert_features_close(g[agree], ref[agree], val, "optimal k=100")
     assert ((g[:, 11] >= 10) & (g[:, 11] <= 100)).all()
+
+
+@pytest.mark.parametrize("off_i,off_d", [(1, 1), (1, 2), (3, 0)])
+def test_search_outputs_at_any_alignment(off_i, off_d):
+    """Index and distance rows are written 16 bytes per store from each
+    row's first 16-byte boundary, one by one when the two arrays are not
+    aligned alike; outputs that start off a 16-byte boundary (views into
+    larger buffers) give the host API's results, for kNN (k = 24, 50),
+    radius search (max 30) and the fused kNN + features path (k = 50, an
+    even k whose rows are written two entries per store when 8-byte aligned)."""
+    data = scenes.lidar_scene(40_000, seed=76, extent=18.0)
+    n = len(data)
+    x = torch.from_numpy(data).to(DEV)
+
+    def view(dtype, shape, off):
+        buf = torch.full((int(np.prod(shape)) + 4,), -7, dtype=dtype, device=DEV)
+        return buf[off:off + int(np.prod(shape))].view(*shape)
+
+    for k in (24, 50):
+        idx, d2 = view(torch.int32, (n, k), off_i), view(torch.float32, (n, k), off_d)
+        capi.knn_search_device(x, x, k, idx, d2)
+        gi, gd = pgeof.knn_search(data, data, k)
+        np.testing.assert_array_equal(idx.cpu().numpy().view(np.uint32), gi)
+        np.testing.assert_array_equal(d2.cpu().numpy(), gd)
+    ridx, rd2 = view(torch.int32, (n, 30), off_i), view(torch.float32, (n, 30), off_d)
+    capi.radius_search_device(x, x, 0.3, 30, ridx, rd2)
+    hi, hd = pgeof.radius_search(data, data, 0.3, 30)
+    np.testing.assert_array_equal(ridx.cpu().numpy(), hi)
+    np.testing.assert_array_equal(rd2.cpu().numpy(), hd)
+    k = 50
+    nn, fd2 = view(torch.int32, (n * k,), off_i), view(torch.float32, (n * k,), off_d)
+    ptr = torch.empty(n + 1, dtype=torch.int32, device=DEV)
+    f = view(torch.float32, (n, 11), off_i)
+    capi.knn_features_device(x, k, nn, ptr, f, sqr_dist=fd2)
+    gi, gd = pgeof.knn_search(data, data, k)
+    np.testing.assert_array_equal(nn.cpu().numpy().view(np.uint32), gi.ravel())
+    np.testing.assert_array_equal(fd2.cpu().numpy(), gd.ravel())
+    nn_h, ptr_h = scenes.knn_csr(gi)
+    np.testing.assert_array_equal(f.cpu().numpy(), pgeof.compute_features(data, nn_h, ptr_h))
