@@ -1774,7 +1774,18 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
             constexpr uint32_t kChunk = 4;
             const uintptr_t aln = reinterpret_cast<uintptr_t>(a.out_idx) | reinterpret_cast<uintptr_t>(a.out_d2);
             const bool vec_out = (k & 3u) == 0 && (aln & 15u) == 0;
-            const bool vec2_out = !vec_out && (k & 1u) == 0 && (aln & 7u) == 0;
+            // k even, rows 8-byte aligned: 16 bytes per store from each row's
+            // first 16-byte boundary when the two arrays are aligned alike (a
+            // row 8 bytes past one stores its first two entries alone, then
+            // carries two entries of every chunk into the next chunk's
+            // store), else 8 bytes per store
+            const bool pair_out = !vec_out && (k & 1u) == 0 && (aln & 7u) == 0;
+            const bool carry_ok =
+                pair_out && (od == nullptr || ((reinterpret_cast<uintptr_t>(oi) ^ reinterpret_cast<uintptr_t>(od)) & 15u) == 0);
+            const bool shifted = carry_ok && (reinterpret_cast<uintptr_t>(oi) & 15u) != 0;
+            uint32_t ci0 = 0, ci1 = 0;
+            float cd0 = 0.0f, cd1 = 0.0f;
+            bool carry = false;
             uint64_t kn[kChunk];
             float4 pn[kChunk];
 #pragma unroll
@@ -1814,7 +1825,44 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
                             __uint_as_float((uint32_t)(kc[0] >> 32)), __uint_as_float((uint32_t)(kc[1] >> 32)),
                             __uint_as_float((uint32_t)(kc[2] >> 32)), __uint_as_float((uint32_t)(kc[3] >> 32)));
                 }
-                else if (vec2_out)
+                else if (carry_ok)
+                {
+                    // k even: entries j0, j0 + 1 exist, and j0 + 2, j0 + 3 together
+                    const uint32_t i0 = __float_as_uint(pc[0].w), i1 = __float_as_uint(pc[1].w);
+                    const uint32_t i2 = __float_as_uint(pc[2].w), i3 = __float_as_uint(pc[3].w);
+                    const float e0 = __uint_as_float((uint32_t)(kc[0] >> 32)), e1 = __uint_as_float((uint32_t)(kc[1] >> 32));
+                    const float e2 = __uint_as_float((uint32_t)(kc[2] >> 32)), e3 = __uint_as_float((uint32_t)(kc[3] >> 32));
+                    const bool full = j0 + 3 < k;
+                    if (!shifted)
+                    {
+                        if (full)
+                        {
+                            *reinterpret_cast<uint4*>(oi + j0) = make_uint4(i0, i1, i2, i3);
+                            if (od) *reinterpret_cast<float4*>(od + j0) = make_float4(e0, e1, e2, e3);
+                        }
+                        else
+                        {
+                            *reinterpret_cast<uint2*>(oi + j0) = make_uint2(i0, i1);
+                            if (od) *reinterpret_cast<float2*>(od + j0) = make_float2(e0, e1);
+                        }
+                    }
+                    else
+                    {
+                        if (j0 == 0)
+                        {
+                            *reinterpret_cast<uint2*>(oi) = make_uint2(i0, i1);
+                            if (od) *reinterpret_cast<float2*>(od) = make_float2(e0, e1);
+                        }
+                        else
+                        {
+                            *reinterpret_cast<uint4*>(oi + j0 - 2) = make_uint4(ci0, ci1, i0, i1);
+                            if (od) *reinterpret_cast<float4*>(od + j0 - 2) = make_float4(cd0, cd1, e0, e1);
+                        }
+                        ci0 = i2, ci1 = i3, cd0 = e2, cd1 = e3;
+                        carry = full;
+                    }
+                }
+                else if (pair_out)
                 {
                     // k even: entries j0, j0 + 1 exist, and j0 + 2, j0 + 3 together
 #pragma unroll
@@ -1842,6 +1890,12 @@ __device__ __forceinline__ void search_slot(const SearchArgs& a, const uint32_t 
 #pragma unroll
                 for (uint32_t u = 0; u < kChunk; ++u)
                     if (j0 + u < k) mom_add(m, (double)pc[u].x - ox, (double)pc[u].y - oy, (double)pc[u].z - oz);
+            }
+            if (carry)
+            {
+                // the row's last two entries (k a multiple of 4, shifted row)
+                *reinterpret_cast<uint2*>(oi + k - 2) = make_uint2(ci0, ci1);
+                if (od) *reinterpret_cast<float2*>(od + k - 2) = make_float2(cd0, cd1);
             }
             double c[6];
             mom_cov(m, (double)k, c);

@@ -260,7 +260,7 @@ def test_optimal_k100_precision_rows():
     assert ((g[:, 11] >= 10) & (g[:, 11] <= 100)).all()
 
 
-@pytest.mark.parametrize("off_i,off_d", [(1, 1), (1, 2), (3, 0)])
+@pytest.mark.parametrize("off_i,off_d", [(0, 0), (1, 1), (1, 2), (3, 0), (2, 2), (2, 0)])
 def test_search_outputs_at_any_alignment(off_i, off_d):
     """Index and distance rows are written 16 bytes per store from each
     row's first 16-byte boundary, one by one when the two arrays are not
