@@ -265,9 +265,10 @@ double pgeof_get_tuning(const char* key);
  * points) with the current knobs and with a sweep of the knobs that move its
  * speed (grid resolution, block size through the heaps' LDS budget, row
  * staging, and for "radius_multiscale" whether the search kernel computes the
- * features), `reps` timed runs each (median), then the four fastest again
- * with 2 * reps + 1 runs each, taken in turns, on `stream`.  The set with the
- * lowest median is kept for the workload's search family ("knn_features" and "knn_optimal":
+ * features), `reps` timed runs each (median), for kNN the fastest of those
+ * also with the other pyramid depth (one level or three), then the four
+ * fastest again with 2 * reps + 1 runs each, taken in turns, on `stream`.
+ * The set with the lowest median is kept for the workload's search family ("knn_features" and "knn_optimal":
  * kNN, knn = k; "radius_multiscale": radius, max_knn = k, scales k_scales)
  * and its size class (ceil(log2 n)): later calls with that k and size class
  * use it, workspace sizes included, until pgeof_autotune_clear.  It allocates
