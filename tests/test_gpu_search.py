@@ -129,6 +129,23 @@ def test_radius_bitexact(r, cap):
         assert (gd[notfull, -1] == np.float32(r) * np.float32(r)).all()
 
 
+@pytest.mark.parametrize("cap", [9, 12, 30])
+def test_radius_rows_of_every_length(cap):
+    """Rows that hold 0, 1, ..., 8 hits (sorted in registers at the end of
+    the search) and rows of 9 or more (heap-sorted), in one search, bit-exact
+    against the oracle; queries partly off the data so that some rows are
+    empty."""
+    rng = np.random.default_rng(77 + cap)
+    data = rng.uniform(0, 4, size=(20_000, 3)).astype(np.float32)
+    q = rng.uniform(-0.3, 4.3, size=(20_000, 3)).astype(np.float32)
+    r = 0.19  # about 9 points per ball inside the data
+    gi, gd = pgeof.radius_search(data, q, r, cap)
+    oi, od, cnt = O.radius(data, q, r, cap, kdtree=True)
+    assert all((cnt == c).any() for c in range(0, 10)), np.bincount(cnt)[:12]
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+
+
 def test_radius_queries_differ():
     rng = np.random.default_rng(13)
     data = rng.uniform(0, 5, size=(6000, 3)).astype(np.float32)
