@@ -841,15 +841,14 @@ int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, in
     a.out = out;
     a.err = err;
     a.order = nullptr;
-    // Plain and optimal-k features on large inputs: visit the rows in the
-    // spatial order of their points (a Morton-code counting sort,
-    // stream-ordered scratch), so the coordinate gathers of a wave hit the
-    // same cache lines.  Measured: optimal k, 1 M rows of 100: 2.7 -> 1.8 ms;
-    // plain, 10 M rows of 20: 3.9 -> 2.6 ms plus 0.8 ms for the order.  The
-    // multiscale kernel gains what the order costs (2 M rows: 2.0 -> 1.8 +
-    // 0.2 ms) and keeps index order.
+    // Features of large inputs: visit the rows in the spatial order of their
+    // points (a Morton-like key sorted in two counting passes, stream-ordered
+    // scratch), so the coordinate gathers of a wave hit the same cache lines.
+    // Measured, order included: optimal k, 1 M rows of 100: 2.7 -> 1.8 ms;
+    // plain, 10 M rows of 20: 3.7-3.8 -> 2.4 ms; multiscale (scales
+    // 10/20/50/100), 2 M radius rows of up to 64: 2.0 -> 1.89 ms.
     ScratchLease tmp;
-    if (mode != 1 && a.n_points >= 65536 && n_xyz > 0 && tuning().csr_order > 0.5)
+    if (a.n_points >= 65536 && n_xyz > 0 && tuning().csr_order > 0.5)
     {
         void* ows = order_ws;
         if (ows)
