@@ -845,7 +845,7 @@ int csr_device(int mode, const float* xyz, int64_t n_xyz, const uint32_t* nn, in
     // points (a Morton-like key sorted in two counting passes, stream-ordered
     // scratch), so the coordinate gathers of a wave hit the same cache lines.
     // Measured, order included: optimal k, 1 M rows of 100: 2.7 -> 1.8 ms;
-    // plain, 10 M rows of 20: 3.7-3.8 -> 2.4 ms; multiscale (scales
+    // plain, 10 M rows of 20: 3.7-3.8 -> 2.3 ms; multiscale (scales
     // 10/20/50/100), 2 M radius rows of up to 64: 2.0 -> 1.89 ms.
     ScratchLease tmp;
     if (a.n_points >= 65536 && n_xyz > 0 && tuning().csr_order > 0.5)

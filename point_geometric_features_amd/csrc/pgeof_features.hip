@@ -269,10 +269,55 @@ __device__ __forceinline__ int longest_axis(const float cell[3])
     return cell[0] >= cell[1] ? (cell[0] >= cell[2] ? 0 : 2) : (cell[1] >= cell[2] ? 1 : 2);
 }
 
+// Where each axis' bits go in the key, from the box alone (the same for
+// every row): bit b handed out (coarsest first) is key bit kOrderBits-1-b,
+// and an axis' bits are handed out from its coordinate's most significant
+// one down, so the key is a bit deposit of each cell coordinate into its
+// axis' mask.
+struct KeyPlan
+{
+    float lo[3], ext[3];
+    int bits[3];
+    uint32_t mask[3];
+};
+
+__device__ __forceinline__ KeyPlan key_plan(const float* __restrict__ bbox)
+{
+    KeyPlan P;
+    float cell[3];
+    for (int ax = 0; ax < 3; ++ax)
+    {
+        P.lo[ax] = bbox[ax];
+        P.ext[ax] = fmaxf(bbox[3 + ax] - bbox[ax], 0.0f);
+        cell[ax] = P.ext[ax];
+        P.bits[ax] = 0;
+        P.mask[ax] = 0;
+    }
+    for (int b = 0; b < kOrderBits; ++b)
+    {
+        const int ax = longest_axis(cell);
+        ++P.bits[ax];
+        P.mask[ax] |= 1u << (kOrderBits - 1 - b);
+        cell[ax] *= 0.5f;
+    }
+    return P;
+}
+
+// the low bits of v, lowest first, into the set bits of mask, lowest first
+__device__ __forceinline__ uint32_t deposit(uint32_t v, uint32_t mask)
+{
+    uint32_t r = 0;
+    for (uint32_t m = mask; m; m &= m - 1u)
+    {
+        if (v & 1u) r |= m & (0u - m);
+        v >>= 1;
+    }
+    return r;
+}
+
 __device__ __forceinline__ uint32_t row_key(const float* __restrict__ xyz, uint32_t n_xyz,
                                             const uint32_t* __restrict__ nn, uint64_t n_nn,
-                                            const uint32_t* __restrict__ nn_ptr, const float* __restrict__ bbox,
-                                            uint32_t i)
+                                            const uint32_t* __restrict__ nn_ptr, const KeyPlan& P, uint32_t i)
 {
     uint32_t j = i;
     if (i >= n_xyz)
@@ -280,40 +325,14 @@ __device__ __forceinline__ uint32_t row_key(const float* __restrict__ xyz, uint3
         const uint32_t b = nn_ptr[i], e = nn_ptr[i + 1];
         j = (b < e && (uint64_t)e <= n_nn) ? nn[b] : n_xyz;
     }
+    if (j >= n_xyz) return 0;
     uint32_t k = 0;
-    if (j < n_xyz)
+    for (int ax = 0; ax < 3; ++ax)
     {
-        float lo[3], ext[3], cell[3];
-        int bits[3] = {0, 0, 0}, used[3] = {0, 0, 0};
-        for (int ax = 0; ax < 3; ++ax)
-        {
-            lo[ax] = bbox[ax];
-            ext[ax] = fmaxf(bbox[3 + ax] - bbox[ax], 0.0f);
-            cell[ax] = ext[ax];
-        }
-        for (int b = 0; b < kOrderBits; ++b)
-        {
-            const int ax = longest_axis(cell);
-            ++bits[ax];
-            cell[ax] *= 0.5f;
-        }
-        uint32_t c[3];
-        for (int ax = 0; ax < 3; ++ax)
-        {
-            const float side = (float)(1u << bits[ax]);
-            float f = ext[ax] > 0.0f ? (xyz[(size_t)j * 3 + ax] - lo[ax]) / ext[ax] * side : 0.0f;
-            f = fminf(fmaxf(f, 0.0f), side - 1.0f);  // also maps NaN to 0
-            c[ax] = (uint32_t)f;
-            cell[ax] = ext[ax];
-        }
-        // the bits in the order they were handed out, coarsest first
-        for (int b = 0; b < kOrderBits; ++b)
-        {
-            const int ax = longest_axis(cell);
-            k = (k << 1) | ((c[ax] >> (bits[ax] - 1 - used[ax])) & 1u);
-            ++used[ax];
-            cell[ax] *= 0.5f;
-        }
+        const float side = (float)(1u << P.bits[ax]);
+        float f = P.ext[ax] > 0.0f ? (xyz[(size_t)j * 3 + ax] - P.lo[ax]) / P.ext[ax] * side : 0.0f;
+        f = fminf(fmaxf(f, 0.0f), side - 1.0f);  // also maps NaN to 0
+        k |= deposit((uint32_t)f, P.mask[ax]);
     }
     return k;
 }
@@ -330,6 +349,7 @@ __global__ __launch_bounds__(kOrderBlock) void k_row_keys(const float* __restric
 {
     __shared__ uint32_t hist[kHiBins];
     for (uint32_t b = threadIdx.x; b < kHiBins; b += kOrderBlock) hist[b] = 0;
+    const KeyPlan P = key_plan(bbox);
     __syncthreads();
 #pragma unroll
     for (uint32_t t = 0; t < kOrderPer; ++t)
@@ -337,7 +357,7 @@ __global__ __launch_bounds__(kOrderBlock) void k_row_keys(const float* __restric
         const uint32_t i = blockIdx.x * kOrderRows + t * kOrderBlock + threadIdx.x;
         if (i < n_points)
         {
-            const uint32_t k = row_key(xyz, n_xyz, nn, n_nn, nn_ptr, bbox, i);
+            const uint32_t k = row_key(xyz, n_xyz, nn, n_nn, nn_ptr, P, i);
             const uint32_t r = atomicAdd(&hist[k >> kLoBits], 1u);
             kr[i] = (k << kRankBits) | r;
         }
