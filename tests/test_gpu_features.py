@@ -248,3 +248,33 @@ def test_row_order_one_bucket():
     finally:
         capi.set_tuning("csr_order", old)
     np.testing.assert_array_equal(res[1], res[0])
+
+
+def test_row_order_more_rows_than_points():
+    """A CSR of more rows than points (the kNN rows of 100,000 queries into
+    40,000 data points, include/pgeof.hpp:75-117 takes the row count from
+    nn_ptr): rows past the end of xyz are ordered by their first neighbour.
+    The features equal index order's bit for bit (plain, multiscale and
+    optimal k) and the oracle's within the parity bar."""
+    from point_geometric_features_amd import capi
+
+    data = scenes.lidar_scene(40_000, seed=36, extent=20.0)
+    q = scenes.lidar_scene(100_000, seed=37, extent=20.0)
+    idx, _ = pgeof.knn_search(data, q, 16)
+    nn, ptr = scenes.knn_csr(idx)
+    old = capi.get_tuning("csr_order")
+    try:
+        res = {}
+        for order in (1, 0):
+            capi.set_tuning("csr_order", order)
+            res[order] = (pgeof.compute_features(data, nn, ptr),
+                          pgeof.compute_features_multiscale(data, nn, ptr, [8, 16]),
+                          pgeof.compute_features_optimal(data, nn, ptr, 8, 2, 8))
+    finally:
+        capi.set_tuning("csr_order", old)
+    for a, b in zip(res[1], res[0]):
+        np.testing.assert_array_equal(a, b)
+    assert res[1][0].shape == (100_000, 11)
+    ref = O.compute_features(data, nn, ptr)
+    val = O.eigvals_csr_np(data, nn, ptr)
+    assert_features_close(res[1][0], ref, val, "more rows than points")
