@@ -45,8 +45,10 @@ static uint64_t poison(size_t q)
 
 // WAVE_SLOTS: the tail of lane l of wave w (64 lanes) at pairs
 // ((w * tail_pairs + p) * 64 + l) * 2, as with one tail per hardware wave
-// slot; otherwise pair p of lane t at (p * lanes + t) * 2.
-template <bool TAIL, bool WAVE_SLOTS = false>
+// slot; otherwise pair p of lane t at (p * lanes + t) * 2.  SHORT: the
+// radius searches' finish (rows of at most 8 keys sorted in registers), with
+// 0..11 keys per lane so that rows below, at and above that size occur.
+template <bool TAIL, bool WAVE_SLOTS = false, bool SHORT = false>
 static int check(std::mt19937_64& rng)
 {
     constexpr uint64_t kGuard = 0x5a5a5a5a5a5a5a5aull;
@@ -86,7 +88,7 @@ static int check(std::mt19937_64& rng)
             }
             heap[t].init(sentinel);
         }
-        const int n = (int)(rng() % 300);
+        const int n = SHORT ? (int)(rng() % 12) : (int)(rng() % 300);
         for (int j = 0; j < n; ++j)
             for (uint32_t t = 0; t < kLanes; ++t)
             {
@@ -97,7 +99,7 @@ static int check(std::mt19937_64& rng)
             }
         for (uint32_t t = 0; t < kLanes; ++t)
         {
-            heap[t].finish(sentinel);
+            heap[t].template finish<SHORT>(sentinel);
             std::vector<uint64_t> want;
             for (uint64_t x : offered[t])
                 if (x < sentinel) want.push_back(x);
@@ -108,8 +110,8 @@ static int check(std::mt19937_64& rng)
                 if (heap[t].key_at(i) != want[i])
                 {
                     if (bad < 10)
-                        std::printf("mismatch: tail=%d slots=%d k=%u lane=%u node=%u\n", (int)TAIL, (int)WAVE_SLOTS,
-                                    k, t, i);
+                        std::printf("mismatch: tail=%d slots=%d short=%d k=%u lane=%u node=%u\n", (int)TAIL,
+                                    (int)WAVE_SLOTS, (int)SHORT, k, t, i);
                     ++bad;
                     break;
                 }
@@ -149,7 +151,8 @@ static int check_tail_offsets()
 int main()
 {
     std::mt19937_64 rng(7);
-    const int bad = check<false>(rng) + check<true>(rng) + check<true, true>(rng) + check_tail_offsets();
+    const int bad = check<false>(rng) + check<true>(rng) + check<true, true>(rng) + check<false, false, true>(rng) +
+                    check<true, false, true>(rng) + check<true, true, true>(rng) + check_tail_offsets();
     std::printf("heap4 check: %d mismatches\n", bad);
     return bad != 0;
 }
